@@ -1,0 +1,167 @@
+/*
+ * rdgpu.h -- C ABI of libCReaDDyGPU.so, the MI355X (gfx950) Brownian-dynamics
+ * step of ReaDDyGPU.
+ *
+ * Every entry point returns 0 on success and a nonzero RD_E* code on error,
+ * as the reference's CudaSimulation methods return 0/1
+ * (CudaSimulation.cuh:40-71).  rd_last_error() gives the message.
+ * Handles are not thread safe; one handle drives one GPU stream.
+ *
+ * Units are whatever the caller uses consistently (the reference GPU input
+ * uses nm, s and kJ/mol; only D*dt and kB*T matter).
+ */
+#ifndef RDGPU_H
+#define RDGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RD_MAX_TYPES 16        /* particle types per system */
+#define RD_MAX_POT2 8          /* order-two potentials per system */
+#define RD_MAX_PARTICLES (1 << 26)
+
+enum {
+    RD_OK = 0,
+    RD_EINVAL = 1,      /* bad argument */
+    RD_ESTATE = 2,      /* call out of order (e.g. step before initialize) */
+    RD_EHIP = 3,        /* HIP runtime error */
+    RD_ERANGE = 4,      /* particle outside the box, non-finite position, capacity */
+    RD_ENOMEM = 5
+};
+
+/* order-one potential kinds: typeID of Simulation.cpp:100-199 */
+enum { RD_POT1_DISK = 1, RD_POT1_CYLINDER = 2, RD_POT1_SPHERE = 3, RD_POT1_BOX = 4 };
+/* order-two potential kinds: HARMONIC, WEAK_INTERACTION_HARMONIC
+ * (Simulation.cpp:172-204) and WEAK_INTERACTION_PIECEWISE_HARMONIC
+ * (P2_WeakInteractionPiecewiseHarmonic.java) */
+enum { RD_POT2_HARMONIC = 1, RD_POT2_WEAK_HARMONIC = 2, RD_POT2_PIECEWISE_WELL = 3 };
+/* subtypeID of Simulation.cpp:105-110, 176-181 */
+enum { RD_SUB_ATTRACTIVE = 1, RD_SUB_REPULSIVE = 2, RD_SUB_SPRING = 3 };
+
+typedef struct rd_system rd_system;
+
+/* Global parameters: Simulation fields read by GlobalParametersHandler.cpp:73-143
+ * and hard-coded in lib/ReaDDyGPU.cpp:96-123. */
+typedef struct {
+    double lattice_bounds[6]; /* latticeBounds: xlo,xhi,ylo,yhi,zlo,zhi (periodic box) */
+    double dt;                /* BD step size (dtOpenMM / stepSizeInPs) */
+    double boltzmann;         /* kB */
+    double temperature;       /* T */
+    double max_cutoff;        /* maxCutoff: pairs with r > max_cutoff are skipped
+                                 (CudaSimulation.cu:1002); reference value 10 */
+    double cell_size;         /* lattice edge; <= 0 chooses 1.01 x the largest
+                                 pair range.  The reference uses maxCutoff. */
+    uint64_t seed;            /* noise seed (Philox key) */
+    uint32_t stream;          /* noise stream (replica id) */
+    int device;               /* cudaDeviceIndex (CudaSimulation.cu:189-200) */
+} rd_params;
+
+/* CudaOrderOnePotential (CudaSimulation.cuh:18-28) plus its row of the
+ * type mask M1[pot][type] (CudaSimulation.cu:286-306). */
+typedef struct {
+    int type;              /* RD_POT1_* */
+    int subtype;           /* RD_SUB_ATTRACTIVE / RD_SUB_REPULSIVE */
+    float force_const;
+    float origin[3];       /* centre (disk, cylinder, sphere) or corner (box) */
+    float normal[3];       /* disk / cylinder axis, any direction */
+    float extension[3];    /* box */
+    float height;          /* cylinder */
+    float radius;          /* disk, cylinder, sphere */
+    uint32_t type_mask;    /* bit t: applies to particle type t */
+} rd_pot1;
+
+/* CudaOrderTwoPotential (CudaSimulation.cuh:30-37). */
+typedef struct {
+    int type;              /* RD_POT2_* */
+    int subtype;           /* RD_SUB_* (harmonic) */
+    float force_const;     /* k */
+    float length;          /* weak harmonic: width; piecewise well: l */
+    float depth;           /* piecewise well: d */
+} rd_pot2;
+
+typedef struct {
+    int n;                 /* live particles */
+    int n_types;
+    int lattice[3];        /* cells per dimension */
+    double cell_size;
+    int stencil;           /* pair stencil radius in cells */
+    uint64_t step;         /* global step counter */
+    long rdf_frames;
+} rd_info;
+
+/* ---- set-up: Simulation::parse (Simulation.cpp:206-280) and
+ *      CudaSimulation::initialize (CudaSimulation.cu:138-455) ---- */
+int rd_create(const rd_params* params, rd_system** out);
+void rd_destroy(rd_system* h);
+const char* rd_last_error(const rd_system* h);
+
+/* particle types: D (ParticleType.D), default radius, and the collision radius
+ * matrix radii[i*nt+j] (0 -> default radius of i).  r0(i,j) = r_i(j) + r_j(i)
+ * as CudaSimulation.cu:357-391. */
+int rd_set_particle_types(rd_system* h, int n_types, const double* D, const double* default_radius,
+                          const double* radii_matrix);
+/* toCudaOrderOnePotential (CudaSimulation.cu:79-117) */
+int rd_add_order_one_potential(rd_system* h, const rd_pot1* pot);
+/* toCudaOrderTwoPotential (CudaSimulation.cu:119-136); pair_mask[i*nt+j]
+ * = M2[i][j][pot] (CudaSimulation.cu:311-353), symmetrised */
+int rd_add_order_two_potential(rd_system* h, const rd_pot2* pot, const uint8_t* pair_mask);
+/* reaction radius per type pair (0: no reaction); candidates are pairs with
+ * dist < R strictly (ReactionObserver.java:101-130) */
+int rd_set_reaction_radii(rd_system* h, const double* radius_matrix);
+/* CoordReader (CoordReader.cpp:16-65) + copyPosToDevice/copyTypesToDevice
+ * (CudaSimulation.cu:905-970).  Particle index = position in these arrays. */
+int rd_set_particles(rd_system* h, int n, const float* xyz, const int* type);
+/* CudaSimulation::initialize: upload tables, build the first cell list */
+int rd_initialize(rd_system* h);
+
+/* ---- the step: Simulation::simulate(int) (Simulation.cpp:349-355) ->
+ *      CudaSimulation::simulate (CudaSimulation.cu:2309-2476) ---- */
+int rd_simulate(rd_system* h, int nsteps);
+
+/* copyPosFromDevice (CudaSimulation.cu:972-984): positions in index order */
+int rd_copy_pos_from_device(rd_system* h, float* xyz, int* type);
+/* reference cell index of every particle, index order: the field formula of
+ * CudaSimulation.cu:586-588 evaluated on the current FP32 positions */
+int rd_cell_ids(rd_system* h, int* field);
+/* gradient of U at the current positions (what orderOne + warpOrderTwo leave
+ * in cudaForces), index order, and the total energy accumulated in FP64 */
+int rd_gradients(rd_system* h, float* grad, double* energy);
+
+/* ---- noise: seeded Philox4x32-10 + Box-Muller, FP32 N(0,1).
+ *      Replaces the curand states of CudaSimulation.cu:749-885, 1921-1926. ---- */
+/* out[(s*3+d)*n + i] = noise used for particle index i, dimension d, step step0+s */
+int rd_noise_fill(rd_system* h, uint64_t step0, int nsteps, float* out);
+/* take noise from the given array (same layout, steps step..step+nsteps-1)
+ * instead of generating it; nsteps = 0 returns to the generator */
+int rd_set_noise_array(rd_system* h, const float* noise, int nsteps);
+
+/* ---- RDF: callRDFCalculation + copyRDFMatrix (CudaSimulation.cu:457-520),
+ *      normalizeRDF (Simulation.cpp:40-62) ---- */
+int rd_rdf_configure(rd_system* h, double r_max, int n_bins);
+int rd_rdf_calculate(rd_system* h);
+/* counts[(ti*nt+tj)*n_bins+bin] over frames so far (ordered pairs) */
+int rd_rdf_copy(rd_system* h, uint64_t* counts, long* frames);
+/* g(r) in FP64 with the shell volume and the box volume */
+int rd_rdf_normalize(rd_system* h, double* g);
+int rd_rdf_reset(rd_system* h);
+
+/* ---- pair sets, index pairs (i<j) sorted lexicographically ---- */
+/* pairs with d^2 < rc^2(ti,tj): the pairs whose interaction is evaluated */
+int rd_neighbor_pairs(rd_system* h, int64_t* out, long cap, long* count);
+/* pairs with d^2 < R^2(ti,tj), R from rd_set_reaction_radii */
+int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long* count);
+
+/* ---- reactions: the cReactions protocol of lib/ReaDDyGPU.cpp:303-463:
+ *      creactions = [count, (readdyId, newType, x, y, z, index) x count];
+ *      index != -1 deletes particle `index`, newType != -1 appends one.
+ *      Survivors keep their order; new particles follow. ---- */
+int rd_apply_reactions(rd_system* h, const float* creactions);
+
+int rd_get_info(rd_system* h, rd_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

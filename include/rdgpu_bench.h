@@ -1,0 +1,29 @@
+/*
+ * rdgpu_bench.h -- measurement entry points of libCReaDDyGPU.so.
+ * These have no counterpart in the reference: they exist for the benchmark
+ * and for profiler calibration.
+ */
+#ifndef RDGPU_BENCH_H
+#define RDGPU_BENCH_H
+#include "rdgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run nsteps steps as one captured graph with HIP events around the
+ * force/update kernel and around the cell sort of every step.
+ * out[0] = host wall time of graph launch + synchronize (ms),
+ * out[1] = first event to last (ms), out[2] = sum of force/update kernel
+ * times, out[3] = sum of cell-sort times. */
+int rd_bench_steps(rd_system* h, int nsteps, double* out);
+
+/* Copy `bytes` bytes device-to-device with a plain vector-copy kernel, reps
+ * times, on the handle's stream (profiler calibration of HBM counters).
+ * ms: average kernel time. */
+int rd_bench_copy(rd_system* h, long bytes, int reps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

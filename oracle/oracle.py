@@ -1,0 +1,219 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker.  See rd_oracle.c for the reference lines each
+function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class orc_pot1(C.Structure):
+    _fields_ = [("kind", C.c_int), ("subtype", C.c_int), ("k", C.c_double), ("origin", C.c_double * 3),
+                ("normal", C.c_double * 3), ("extension", C.c_double * 3), ("height", C.c_double),
+                ("radius", C.c_double), ("type_mask", C.c_uint64)]
+
+
+class orc_pot2(C.Structure):
+    _fields_ = [("kind", C.c_int), ("subtype", C.c_int), ("k", C.c_double), ("length", C.c_double),
+                ("depth", C.c_double), ("cutoff", C.c_double), ("mask", C.c_void_p)]
+
+
+class orc_system(C.Structure):
+    _fields_ = [("n_types", C.c_int), ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("cell", C.c_double),
+                ("kbt", C.c_double), ("dt", C.c_double), ("D", C.c_void_p), ("prad", C.c_void_p),
+                ("r0", C.c_void_p), ("rc2", C.c_void_p), ("n_pot1", C.c_int), ("pot1", C.c_void_p),
+                ("n_pot2", C.c_int), ("pot2", C.c_void_p), ("seed", C.c_uint64), ("stream", C.c_uint32)]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        V = C.c_void_p
+        sig = {
+            "orc_philox4x32_10": (None, [V, V, V]),
+            "orc_log_f32": (C.c_float, [C.c_float]),
+            "orc_sincos_turns_f32": (None, [C.c_float, V, V]),
+            "orc_normal3": (None, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, V]),
+            "orc_noise_fill": (None, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int, C.c_int, V]),
+            "orc_lattice_dims": (None, [V, V]),
+            "orc_cell_index": (None, [V, C.c_int, V, V, V]),
+            "orc_pair_eval": (C.c_double, [V, C.c_double, V, V]),
+            "orc_ext_eval": (C.c_double, [V, C.c_double, V, V]),
+            "orc_disk_java": (C.c_double, [V, C.c_double, V, V]),
+            "orc_gradients_bruteforce": (C.c_double, [V, C.c_int, V, V, V]),
+            "orc_gradients_cells": (C.c_double, [V, C.c_int, V, V, V, C.c_int]),
+            "orc_run": (C.c_int, [V, C.c_int, V, V, C.c_uint64, C.c_int, V, C.c_int]),
+            "orc_pairs_bruteforce": (C.c_long, [V, C.c_int, V, V, V, V, C.c_long]),
+            "orc_pairs_cells": (C.c_long, [V, C.c_int, V, V, V, V, C.c_long]),
+            "orc_rdf_counts": (None, [V, C.c_int, V, V, C.c_float, C.c_int, V, C.c_int]),
+            "orc_rdf_normalize": (None, [C.c_int, C.c_int, C.c_double, C.c_double, V, C.c_long, V, V]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def make_pot1(kind, subtype, k, origin=(0, 0, 0), normal=(0, 0, 1), extension=(0, 0, 0), height=0.0,
+              radius=0.0, type_mask=~0 & 0xFFFFFFFFFFFFFFFF):
+    p = orc_pot1()
+    p.kind, p.subtype, p.k = kind, subtype, k
+    for d in range(3):
+        p.origin[d], p.normal[d], p.extension[d] = origin[d], normal[d], extension[d]
+    p.height, p.radius, p.type_mask = height, radius, type_mask
+    return p
+
+
+def make_pot2(kind, subtype, k, length=0.0, depth=0.0, cutoff=1e30):
+    p = orc_pot2()
+    p.kind, p.subtype, p.k, p.length, p.depth, p.cutoff = kind, subtype, k, length, depth, cutoff
+    p.mask = None
+    return p
+
+
+class OracleSystem:
+    """orc_system built from a readdygpu_amd.model.SystemSpec."""
+
+    def __init__(self, spec, cell=None):
+        self.spec = spec
+        nt = spec.n_types
+        s = orc_system()
+        s.n_types = nt
+        for d in range(3):
+            s.lo[d], s.hi[d] = spec.box_lo[d], spec.box_hi[d]
+        rc = spec.pair_range()
+        rx = 0.0 if spec.reaction_radii is None else float(np.max(spec.reaction_radii))
+        if cell is None:
+            m = max(float(rc.max()), rx)
+            cell = spec.cell_size if spec.cell_size > 0 else (1.01 * m if m > 0 else spec.max_cutoff)
+        s.cell = float(np.float32(cell))
+        s.kbt = spec.kbt
+        s.dt = spec.dt
+        self._D = np.array([t.D for t in spec.types], dtype=np.float64)
+        self._prad = np.array([t.radius for t in spec.types], dtype=np.float64)
+        self._r0 = np.ascontiguousarray(spec.radius_sum(), dtype=np.float64)
+        self._rc2 = np.ascontiguousarray(spec.rc2_f32())
+        self.rr2 = np.ascontiguousarray(spec.rr2_f32())
+        s.D, s.prad, s.r0, s.rc2 = _p(self._D), _p(self._prad), _p(self._r0), _p(self._rc2)
+        p1 = (orc_pot1 * max(len(spec.pot1), 1))()
+        for i, q in enumerate(spec.pot1):
+            p1[i] = make_pot1(q.kind, q.subtype, q.k, q.origin, q.normal, q.extension, q.height, q.radius,
+                              spec.pot1_mask(q))
+        p2 = (orc_pot2 * max(len(spec.pot2), 1))()
+        self._masks = []
+        for i, q in enumerate(spec.pot2):
+            m = np.ascontiguousarray(spec.pot2_mask(q), dtype=np.uint8)
+            self._masks.append(m)
+            p2[i] = make_pot2(q.kind, q.subtype, q.k, q.length, q.depth, spec.max_cutoff)
+            p2[i].mask = _p(m)
+        self._p1, self._p2 = p1, p2
+        s.n_pot1, s.pot1 = len(spec.pot1), C.cast(p1, C.c_void_p)
+        s.n_pot2, s.pot2 = len(spec.pot2), C.cast(p2, C.c_void_p)
+        s.seed, s.stream = spec.seed, spec.stream
+        self.s = s
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+    def lattice(self):
+        L = np.zeros(3, dtype=np.int32)
+        lib().orc_lattice_dims(self.ref, _p(L))
+        return L
+
+    def cell_index(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        n = len(xyz)
+        f = np.empty(n, dtype=np.int32)
+        c = np.empty((n, 3), dtype=np.int32)
+        lib().orc_cell_index(self.ref, n, _p(xyz), _p(f), _p(c))
+        return f, c
+
+    def gradients(self, xyz, types, brute=False, nthreads=0):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        g = np.empty_like(xyz)
+        if brute:
+            e = lib().orc_gradients_bruteforce(self.ref, len(types), _p(xyz), _p(types), _p(g))
+        else:
+            e = lib().orc_gradients_cells(self.ref, len(types), _p(xyz), _p(types), _p(g), nthreads)
+        return g, e
+
+    def run(self, xyz, types, step0, nsteps, noise=None, nthreads=0):
+        x = np.ascontiguousarray(xyz, dtype=np.float64).copy()
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float32)
+        bad = lib().orc_run(self.ref, len(types), _p(x), _p(types), int(step0), int(nsteps), _p(nz), nthreads)
+        return x, bad
+
+    def pairs(self, xyz, types, thr2=None, brute=False):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        t = None if thr2 is None else np.ascontiguousarray(thr2, dtype=np.float32)
+        fn = lib().orc_pairs_bruteforce if brute else lib().orc_pairs_cells
+        cnt = fn(self.ref, len(types), _p(xyz), _p(types), _p(t), None, 0)
+        out = np.empty((max(cnt, 1), 2), dtype=np.int64)
+        fn(self.ref, len(types), _p(xyz), _p(types), _p(t), _p(out), cnt)
+        out = out[:cnt]
+        if cnt:
+            out = out[np.lexsort((out[:, 1], out[:, 0]))]
+        return out
+
+    def candidates(self, xyz, types, brute=False):
+        return self.pairs(xyz, types, thr2=self.rr2, brute=brute)
+
+    def rdf_counts(self, xyz, types, rmax, nbins, cells=True):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        nt = self.spec.n_types
+        c = np.zeros((nt, nt, nbins), dtype=np.uint64)
+        lib().orc_rdf_counts(self.ref, len(types), _p(xyz), _p(types), float(rmax), int(nbins), _p(c),
+                             1 if cells else 0)
+        return c
+
+
+def noise(seed, stream, step0, nsteps, n):
+    out = np.empty((nsteps, 3, n), dtype=np.float32)
+    lib().orc_noise_fill(int(seed), int(stream), int(step0), int(nsteps), int(n), _p(out))
+    return out
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.empty(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(o))
+    return o
+
+
+def rdf_normalize(nt, nbins, rmax, volume, npt, frames, counts):
+    npt = np.ascontiguousarray(npt, dtype=np.int32)
+    counts = np.ascontiguousarray(counts, dtype=np.uint64)
+    g = np.zeros(counts.shape, dtype=np.float64)
+    lib().orc_rdf_normalize(nt, nbins, float(rmax), float(volume), _p(npt), int(frames), _p(counts), _p(g))
+    return g

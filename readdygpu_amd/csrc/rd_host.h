@@ -1,0 +1,83 @@
+// rd_host.h -- host-side state of one system (the CudaSimulation object of
+// CudaSimulation.cuh:39-116, re-laid out for the sorted SoA design).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rdgpu.h"
+#include "rd_device.h"
+
+struct RdStepState;
+
+struct rd_system {
+    rd_params prm{};
+    std::string err;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // model (Simulation fields, Simulation.hpp:138-181)
+    int nt = 0;
+    std::vector<double> D, rad, radii, rxn;
+    std::vector<rd_pot1> pot1;
+    std::vector<rd_pot2> pot2;
+    std::vector<std::vector<uint8_t>> pot2mask;
+    std::vector<RdPot1Dev> pot1dev;
+    RdDev P{};
+    double cell = 0.0, max_range = 0.0, max_rxn = 0.0;
+    int ncell = 0;
+    int nscan = 0;
+
+    // particles in index order (host mirror)
+    int n = 0;
+    std::vector<float> hxyz;
+    std::vector<int> htype;
+
+    bool initialized = false;
+    bool dirty = true;            // tables changed
+    bool particles_dirty = true;  // host arrays newer than device
+    uint64_t step = 0;
+
+    // device buffers
+    float4* d_ps = nullptr;
+    float4* d_pn = nullptr;
+    uint32_t* d_key = nullptr;
+    uint32_t* d_rank = nullptr;
+    uint32_t* d_count = nullptr;
+    uint32_t* d_start = nullptr;
+    uint32_t* d_bsum = nullptr;
+    RdDev* d_P = nullptr;
+    RdPot1Dev* d_pot1 = nullptr;
+    RdStepState* d_st = nullptr;
+    float* d_noise = nullptr;
+    float* d_xyz_io = nullptr;
+    int* d_type_io = nullptr;
+    double* d_etot = nullptr;
+    int cap = 0;
+    int ncell_alloc = 0;
+    size_t noise_cap = 0;
+    int noise_n = -1;             // particle count the noise array was laid out for
+    size_t io_cap = 0;
+
+    // RDF
+    unsigned long long* d_rdf = nullptr;
+    double rdf_rmax = 0.0;
+    int rdf_bins = 0;
+    long rdf_frames = 0;
+    double rdf_vol = 1.0;
+    std::vector<double> rdf_npt_sum, rdf_pairs_sum;
+
+    // pair emission
+    uint2* d_pairs = nullptr;
+    unsigned long long* d_pcount = nullptr;
+    size_t pairs_cap = 0;
+
+    // captured step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+double rd_host_r0(const rd_system* h, int i, int j);

@@ -1,0 +1,1091 @@
+// rd_system.hip -- the plain (single system) BD step behind the rd_* C ABI.
+//
+// Host side mirrors CudaSimulation (CudaSimulation.cuh:39-116): set-up,
+// step, transfers, RDF; the kernels are new gfx950 designs (see rd_kernels.h).
+#include "rd_kernels.h"
+#include "rd_host.h"
+
+#include <algorithm>
+#include <cstddef>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+
+// Pack unsorted host input into slot form and histogram the cells.
+__global__ __launch_bounds__(RD_BLOCK) void k_pack(const float* __restrict__ xyz, const int* __restrict__ type, int n,
+                                                   const RdDev* __restrict__ P, float4* __restrict__ pn,
+                                                   uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                   uint32_t* __restrict__ count)
+{
+    int i = blockIdx.x * RD_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    pn[i] = make_float4(x, y, z, w_pack((uint32_t)i, type[i]));
+    uint32_t k = cell_key(P, x, y, z);
+    key[i] = k;
+    rank[i] = atomicAdd(&count[k], 1u);
+}
+
+// One BD step for the particle in slot s (CudaSimulation.cu:2314-2465 fused):
+// gradient gather, Euler-Maruyama x += -a*dU + b*xi (CudaSimulation.cu:1852),
+// periodic wrap (:1857-1858), next cell key and its histogram.
+__global__ __launch_bounds__(RD_BLOCK) void k_force_update(const float4* __restrict__ ps, float4* __restrict__ pn,
+                                                           uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                           uint32_t* __restrict__ count,
+                                                           const uint32_t* __restrict__ start,
+                                                           const RdDev* __restrict__ P,
+                                                           const RdPot1Dev* __restrict__ pot1,
+                                                           RdStepState* __restrict__ st,
+                                                           const float* __restrict__ noise, int n)
+{
+    int b = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t s = (uint32_t)b * RD_BLOCK + threadIdx.x;
+    if (s >= (uint32_t)n) return;
+    float4 p = ps[s];
+    float3 g = gather_grad<false>(P, pot1, ps, start, s, p, nullptr);
+    const int t = w_type(p.w);
+    const uint32_t idx = w_index(p.w);
+    const unsigned long long step = st->step;
+    float3 z;
+    if (st->noise_nsteps == 0) {
+        z = rd_normal3(P->seed, P->stream, step, idx);
+    } else {
+        long long ls = (long long)(step - st->noise_step0);
+        if (ls >= 0 && ls < st->noise_nsteps) {
+            size_t base = (size_t)ls * 3 * (size_t)n + idx;
+            z = make_float3(noise[base], noise[base + n], noise[base + 2 * (size_t)n]);
+        } else {
+            z = make_float3(0.0f, 0.0f, 0.0f);
+            atomicOr(&st->err, 2u);
+        }
+    }
+    const float a = P->a[t], bb = P->b[t];
+    float x = (p.x - a * g.x) + bb * z.x;
+    float y = (p.y - a * g.y) + bb * z.y;
+    float zc = (p.z - a * g.z) + bb * z.z;
+    x = wrap_coord(x, P->lo[0], P->hi[0], P->Lb[0]);
+    y = wrap_coord(y, P->lo[1], P->hi[1], P->Lb[1]);
+    zc = wrap_coord(zc, P->lo[2], P->hi[2], P->Lb[2]);
+    if (!(fabsf(x) < 3.0e38f && fabsf(y) < 3.0e38f && fabsf(zc) < 3.0e38f)) atomicOr(&st->err, 1u);
+    pn[s] = make_float4(x, y, zc, p.w);
+    uint32_t k = cell_key(P, x, y, zc);
+    key[s] = k;
+    rank[s] = atomicAdd(&count[k], 1u);
+}
+
+// ---- exclusive scan of count[ncell] -> start[ncell+1], zeroing count ----
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// block-wide exclusive scan of one value per thread; returns the block total
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* excl, uint32_t* lds)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) lds[wid] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < RD_BLOCK / 64; ++w) {
+        uint32_t t = lds[w];
+        if (w < wid) off += t;
+        tot += t;
+    }
+    *excl = off + inc - v;
+    __syncthreads();
+    return tot;
+}
+
+__global__ __launch_bounds__(RD_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ count, int ncell,
+                                                          uint32_t* __restrict__ bsum)
+{
+    __shared__ uint32_t lds[RD_BLOCK / 64];
+    size_t base = (size_t)blockIdx.x * RD_SCAN_TILE + (size_t)threadIdx.x * RD_SCAN_ITEMS;
+    uint32_t v = 0;
+    if (base + RD_SCAN_ITEMS <= (size_t)ncell) {
+        const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
+#pragma unroll
+        for (int k = 0; k < RD_SCAN_ITEMS / 4; ++k) {
+            uint4 q = c4[k];
+            v += q.x + q.y + q.z + q.w;
+        }
+    } else {
+        for (int k = 0; k < RD_SCAN_ITEMS; ++k)
+            if (base + k < (size_t)ncell) v += count[base + k];
+    }
+    uint32_t ex;
+    uint32_t tot = block_excl_scan(v, &ex, lds);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of the tile sums; writes start[ncell] = total
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* __restrict__ bsum, int nb,
+                                                    uint32_t* __restrict__ start, int ncell)
+{
+    __shared__ uint32_t lds[16];
+    __shared__ uint32_t carry;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        int i = base + threadIdx.x;
+        uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t inc = wave_incl_scan(v);
+        if (lane == 63) lds[wid] = inc;
+        __syncthreads();
+        uint32_t off = carry, tot = 0;
+        for (int w = 0; w < 16; ++w) {
+            uint32_t t = lds[w];
+            if (w < wid) off += t;
+            tot += t;
+        }
+        if (i < nb) bsum[i] = off + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[ncell] = carry;
+}
+
+__global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ count, int ncell,
+                                                         const uint32_t* __restrict__ bsum,
+                                                         uint32_t* __restrict__ start)
+{
+    __shared__ uint32_t lds[RD_BLOCK / 64];
+    size_t base = (size_t)blockIdx.x * RD_SCAN_TILE + (size_t)threadIdx.x * RD_SCAN_ITEMS;
+    uint32_t c[RD_SCAN_ITEMS];
+    bool full = base + RD_SCAN_ITEMS <= (size_t)ncell;
+    if (full) {
+        const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
+#pragma unroll
+        for (int k = 0; k < RD_SCAN_ITEMS / 4; ++k) {
+            uint4 q = c4[k];
+            c[4 * k] = q.x; c[4 * k + 1] = q.y; c[4 * k + 2] = q.z; c[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < RD_SCAN_ITEMS; ++k) c[k] = (base + k < (size_t)ncell) ? count[base + k] : 0u;
+    }
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < RD_SCAN_ITEMS; ++k) v += c[k];
+    uint32_t ex;
+    block_excl_scan(v, &ex, lds);
+    uint32_t run = bsum[blockIdx.x] + ex;
+    if (full) {
+        uint4* s4 = reinterpret_cast<uint4*>(start + base);
+        uint4* z4 = reinterpret_cast<uint4*>(count + base);
+#pragma unroll
+        for (int k = 0; k < RD_SCAN_ITEMS / 4; ++k) {
+            uint4 o;
+            o.x = run; run += c[4 * k];
+            o.y = run; run += c[4 * k + 1];
+            o.z = run; run += c[4 * k + 2];
+            o.w = run; run += c[4 * k + 3];
+            s4[k] = o;
+            z4[k] = make_uint4(0u, 0u, 0u, 0u);
+        }
+    } else {
+        for (int k = 0; k < RD_SCAN_ITEMS; ++k)
+            if (base + k < (size_t)ncell) { start[base + k] = run; run += c[k]; count[base + k] = 0u; }
+    }
+}
+
+// counting-sort scatter; advances the step counter when `advance` is set
+__global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__ pn, const uint32_t* __restrict__ key,
+                                                      const uint32_t* __restrict__ rank,
+                                                      const uint32_t* __restrict__ start, float4* __restrict__ ps,
+                                                      int n, RdStepState* __restrict__ st, int advance)
+{
+    int s = blockIdx.x * RD_BLOCK + threadIdx.x;
+    if (advance && s == 0) st->step += 1ull;
+    if (s >= n) return;
+    uint32_t dst = start[key[s]] + rank[s];
+    ps[dst] = pn[s];
+}
+
+// positions back to index order
+__global__ __launch_bounds__(RD_BLOCK) void k_unsort(const float4* __restrict__ ps, int n, float* __restrict__ xyz,
+                                                     int* __restrict__ type)
+{
+    int s = blockIdx.x * RD_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    float4 p = ps[s];
+    uint32_t i = w_index(p.w);
+    xyz[3 * i] = p.x; xyz[3 * i + 1] = p.y; xyz[3 * i + 2] = p.z;
+    type[i] = w_type(p.w);
+}
+
+// reference cell index per particle index
+__global__ __launch_bounds__(RD_BLOCK) void k_cell_ids(const float4* __restrict__ ps, int n,
+                                                       const RdDev* __restrict__ P, int* __restrict__ field)
+{
+    int s = blockIdx.x * RD_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    float4 p = ps[s];
+    field[w_index(p.w)] = (int)cell_key(P, p.x, p.y, p.z);
+}
+
+// gradient + energy (FP64 accumulation) at the current positions
+__global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict__ ps,
+                                                        const uint32_t* __restrict__ start,
+                                                        const RdDev* __restrict__ P,
+                                                        const RdPot1Dev* __restrict__ pot1, int n,
+                                                        float* __restrict__ grad, double* __restrict__ etot)
+{
+    __shared__ double red[RD_BLOCK / 64];
+    uint32_t s = blockIdx.x * RD_BLOCK + threadIdx.x;
+    double e = 0.0;
+    if (s < (uint32_t)n) {
+        float4 p = ps[s];
+        float ef = 0.0f;
+        float3 g = gather_grad<true>(P, pot1, ps, start, s, p, &ef);
+        uint32_t i = w_index(p.w);
+        grad[3 * i] = g.x; grad[3 * i + 1] = g.y; grad[3 * i + 2] = g.z;
+        e = (double)ef;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < RD_BLOCK / 64; ++w) t += red[w];
+        atomicAdd(etot, t);
+    }
+}
+
+// noise arrays out[(s*3+d)*n + i]
+__global__ __launch_bounds__(RD_BLOCK) void k_noise_fill(const RdDev* __restrict__ P, unsigned long long step0,
+                                                         int nsteps, int n, float* __restrict__ out)
+{
+    size_t g = (size_t)blockIdx.x * RD_BLOCK + threadIdx.x;
+    size_t total = (size_t)nsteps * (size_t)n;
+    if (g >= total) return;
+    int s = (int)(g / (size_t)n);
+    int i = (int)(g - (size_t)s * n);
+    float3 z = rd_normal3(P->seed, P->stream, step0 + (unsigned long long)s, (uint32_t)i);
+    size_t base = (size_t)s * 3 * (size_t)n + i;
+    out[base] = z.x; out[base + n] = z.y; out[base + 2 * (size_t)n] = z.z;
+}
+
+// RDF histogram, privatised in LDS (CudaSimulation.cu:1784-1827 counts every
+// ordered pair once per side; here each slot counts its own partners).
+template <bool LDS>
+__global__ __launch_bounds__(RD_BLOCK) void k_rdf(const float4* __restrict__ ps, const uint32_t* __restrict__ start,
+                                                  const RdDev* __restrict__ P, int S, float rmax, int nbins, int n,
+                                                  unsigned long long* __restrict__ counts)
+{
+    extern __shared__ uint32_t hist[];
+    const int nt = P->nt;
+    const int nh = nt * nt * nbins;
+    if (LDS) {
+        for (int k = threadIdx.x; k < nh; k += RD_BLOCK) hist[k] = 0u;
+        __syncthreads();
+    }
+    int b = xcd_block(blockIdx.x, gridDim.x);
+    uint32_t s = (uint32_t)b * RD_BLOCK + threadIdx.x;
+    if (s < (uint32_t)n) {
+        float4 p = ps[s];
+        int ti = w_type(p.w);
+        int cx = cell_coord(p.x, P->lo[0], P->cell, P->L[0]);
+        int cy = cell_coord(p.y, P->lo[1], P->cell, P->L[1]);
+        int cz = cell_coord(p.z, P->lo[2], P->cell, P->L[2]);
+        const float fnb = (float)nbins;
+        for_stencil(P, start, cx, cy, cz, S, [&](uint32_t j) {
+            if (j == s) return;
+            float4 q = ps[j];
+            float dx = min_image(p.x - q.x, P->Lb[0], P->half[0]);
+            float dy = min_image(p.y - q.y, P->Lb[1], P->half[1]);
+            float dz = min_image(p.z - q.z, P->Lb[2], P->half[2]);
+            float d2 = dx * dx;
+            d2 = d2 + dy * dy;
+            d2 = d2 + dz * dz;
+            float r = sqrtf(d2);
+            if (r < rmax) {
+                int bin = (int)floorf((r * fnb) / rmax);
+                if (bin > nbins - 1) bin = nbins - 1;
+                int h = (ti * nt + w_type(q.w)) * nbins + bin;
+                if (LDS) atomicAdd(&hist[h], 1u);
+                else atomicAdd(&counts[h], 1ull);
+            }
+        });
+    }
+    if (LDS) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < nh; k += RD_BLOCK) {
+            uint32_t v = hist[k];
+            if (v) atomicAdd(&counts[k], (unsigned long long)v);
+        }
+    }
+}
+
+// pairs (i<j by index) with d2 < thr2[ti][tj]; wave-aggregated append
+__global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ ps, const uint32_t* __restrict__ start,
+                                                    const RdDev* __restrict__ P, const float* __restrict__ thr2,
+                                                    int S, int n, uint2* __restrict__ out, unsigned long long cap,
+                                                    unsigned long long* __restrict__ counter)
+{
+    uint32_t s = blockIdx.x * RD_BLOCK + threadIdx.x;
+    if (s >= (uint32_t)n) return;
+    float4 p = ps[s];
+    const int nt = P->nt;
+    int ti = w_type(p.w);
+    uint32_t ii = w_index(p.w);
+    int cx = cell_coord(p.x, P->lo[0], P->cell, P->L[0]);
+    int cy = cell_coord(p.y, P->lo[1], P->cell, P->L[1]);
+    int cz = cell_coord(p.z, P->lo[2], P->cell, P->L[2]);
+    for_stencil(P, start, cx, cy, cz, S, [&](uint32_t j) {
+        float4 q = ps[j];
+        uint32_t jj = w_index(q.w);
+        if (jj <= ii) return;
+        float dx = min_image(p.x - q.x, P->Lb[0], P->half[0]);
+        float dy = min_image(p.y - q.y, P->Lb[1], P->half[1]);
+        float dz = min_image(p.z - q.z, P->Lb[2], P->half[2]);
+        float d2 = dx * dx;
+        d2 = d2 + dy * dy;
+        d2 = d2 + dz * dz;
+        if (d2 < thr2[ti * nt + w_type(q.w)]) {
+            unsigned long long slot = atomicAdd(counter, 1ull);
+            if (slot < cap) out[slot] = make_uint2(ii, jj);
+        }
+    });
+}
+
+// ===========================================================================
+// Host runtime
+// ===========================================================================
+
+static int set_err(rd_system* h, int code, const std::string& m)
+{
+    if (h) h->err = m;
+    return code;
+}
+
+#define HIPCK(h, x)                                                                            \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            return set_err((h), RD_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+static void free_dev(rd_system* h)
+{
+    void* ptrs[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_bsum,
+                     h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_pairs, h->d_pcount,
+                     h->d_xyz_io, h->d_type_io, h->d_etot };
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_ps = h->d_pn = nullptr;
+    h->d_key = h->d_rank = h->d_count = h->d_start = h->d_bsum = nullptr;
+    h->d_P = nullptr; h->d_pot1 = nullptr; h->d_st = nullptr; h->d_noise = nullptr;
+    h->d_rdf = nullptr; h->d_pairs = nullptr; h->d_pcount = nullptr;
+    h->d_xyz_io = nullptr; h->d_type_io = nullptr; h->d_etot = nullptr;
+    h->cap = 0; h->ncell_alloc = 0; h->pairs_cap = 0; h->noise_cap = 0; h->io_cap = 0;
+}
+
+static void drop_graph(rd_system* h)
+{
+    if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+}
+
+extern "C" int rd_create(const rd_params* prm, rd_system** out)
+{
+    if (!prm || !out) return RD_EINVAL;
+    *out = nullptr;
+    for (int d = 0; d < 3; ++d)
+        if (!(prm->lattice_bounds[2 * d + 1] > prm->lattice_bounds[2 * d])) return RD_EINVAL;
+    if (!(prm->dt > 0) || !(prm->boltzmann > 0) || !(prm->temperature > 0) || !(prm->max_cutoff > 0))
+        return RD_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return RD_EHIP;
+    rd_system* h = new rd_system();
+    h->prm = *prm;
+    h->dev = (prm->device >= 0 && prm->device < ndev) ? prm->device : 0;
+    if (hipSetDevice(h->dev) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return RD_EHIP;
+    }
+    h->step = 0;
+    *out = h;
+    return RD_OK;
+}
+
+extern "C" void rd_destroy(rd_system* h)
+{
+    if (!h) return;
+    (void)hipSetDevice(h->dev);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    free_dev(h);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    delete h;
+}
+
+extern "C" const char* rd_last_error(const rd_system* h)
+{
+    return h ? h->err.c_str() : "null handle";
+}
+
+extern "C" int rd_set_particle_types(rd_system* h, int nt, const double* D, const double* rad, const double* radii)
+{
+    if (!h) return RD_EINVAL;
+    if (nt < 1 || nt > RD_MAX_TYPES || !D || !rad) return set_err(h, RD_EINVAL, "bad particle types");
+    h->nt = nt;
+    h->D.assign(D, D + nt);
+    h->rad.assign(rad, rad + nt);
+    h->radii.assign((size_t)nt * nt, 0.0);
+    if (radii) h->radii.assign(radii, radii + (size_t)nt * nt);
+    h->rxn.assign((size_t)nt * nt, 0.0);
+    h->dirty = true;
+    return RD_OK;
+}
+
+extern "C" int rd_add_order_one_potential(rd_system* h, const rd_pot1* p)
+{
+    if (!h || !p) return RD_EINVAL;
+    if (p->type < RD_POT1_DISK || p->type > RD_POT1_BOX) return set_err(h, RD_EINVAL, "unknown order-one potential");
+    h->pot1.push_back(*p);
+    h->dirty = true;
+    return RD_OK;
+}
+
+extern "C" int rd_add_order_two_potential(rd_system* h, const rd_pot2* p, const uint8_t* mask)
+{
+    if (!h || !p || !mask) return RD_EINVAL;
+    if (h->nt == 0) return set_err(h, RD_ESTATE, "set particle types first");
+    if ((int)h->pot2.size() >= RD_MAX_POT2) return set_err(h, RD_EINVAL, "too many order-two potentials");
+    if (p->type < RD_POT2_HARMONIC || p->type > RD_POT2_PIECEWISE_WELL)
+        return set_err(h, RD_EINVAL, "unknown order-two potential");
+    if (p->type == RD_POT2_PIECEWISE_WELL && !(p->length > 0)) return set_err(h, RD_EINVAL, "well length must be > 0");
+    h->pot2.push_back(*p);
+    std::vector<uint8_t> m((size_t)h->nt * h->nt, 0);
+    for (int i = 0; i < h->nt; ++i)
+        for (int j = 0; j < h->nt; ++j)
+            if (mask[i * h->nt + j]) { m[i * h->nt + j] = 1; m[j * h->nt + i] = 1; }
+    h->pot2mask.push_back(m);
+    h->dirty = true;
+    return RD_OK;
+}
+
+extern "C" int rd_set_reaction_radii(rd_system* h, const double* R)
+{
+    if (!h || !R) return RD_EINVAL;
+    if (h->nt == 0) return set_err(h, RD_ESTATE, "set particle types first");
+    h->rxn.assign(R, R + (size_t)h->nt * h->nt);
+    for (int i = 0; i < h->nt; ++i)
+        for (int j = 0; j < h->nt; ++j) {
+            double v = std::max(h->rxn[i * h->nt + j], h->rxn[j * h->nt + i]);
+            h->rxn[i * h->nt + j] = h->rxn[j * h->nt + i] = v;
+        }
+    h->dirty = true;
+    return RD_OK;
+}
+
+// Range of the evaluated interaction for a type pair: the largest natural
+// range of the potentials that apply, capped at maxCutoff.
+static double pair_range(const rd_system* h, int i, int j, double r0)
+{
+    double m = 0.0;
+    for (size_t p = 0; p < h->pot2.size(); ++p) {
+        if (!h->pot2mask[p][i * h->nt + j]) continue;
+        const rd_pot2& q = h->pot2[p];
+        double r = 0.0;
+        if (q.type == RD_POT2_HARMONIC) r = (q.subtype == RD_SUB_REPULSIVE) ? r0 : h->prm.max_cutoff;
+        else r = r0 + q.length;
+        m = std::max(m, std::min(r, h->prm.max_cutoff));
+    }
+    return m;
+}
+
+double rd_host_r0(const rd_system* h, int i, int j)
+{
+    int nt = h->nt;
+    double ri = h->radii[i * nt + j] == 0.0 ? h->rad[i] : h->radii[i * nt + j];
+    double rj = h->radii[j * nt + i] == 0.0 ? h->rad[j] : h->radii[j * nt + i];
+    return ri + rj;
+}
+
+// Build the device constant block (CudaSimulation.cu:237-391 tables).
+static int build_tables(rd_system* h)
+{
+    RdDev& P = h->P;
+    memset(&P, 0, sizeof(P));
+    const rd_params& g = h->prm;
+    int nt = h->nt;
+    P.nt = nt;
+    for (int d = 0; d < 3; ++d) {
+        P.lo[d] = (float)g.lattice_bounds[2 * d];
+        P.hi[d] = (float)g.lattice_bounds[2 * d + 1];
+        P.Lb[d] = P.hi[d] - P.lo[d];
+        P.half[d] = P.Lb[d] / 2.0f;
+    }
+    double maxr = 0.0, maxrx = 0.0;
+    for (int i = 0; i < nt; ++i)
+        for (int j = 0; j < nt; ++j) {
+            double r0 = rd_host_r0(h, i, j);
+            double rc = pair_range(h, i, j, r0);
+            P.r0[i * nt + j] = (float)r0;
+            P.rc2[i * nt + j] = (float)(rc * rc);
+            P.rr2[i * nt + j] = (float)(h->rxn[i * nt + j] * h->rxn[i * nt + j]);
+            maxr = std::max(maxr, rc);
+            maxrx = std::max(maxrx, h->rxn[i * nt + j]);
+            uint32_t m = 0;
+            for (size_t p = 0; p < h->pot2.size(); ++p)
+                if (h->pot2mask[p][i * nt + j]) m |= 1u << p;
+            P.p2mask[i * nt + j] = m;
+        }
+    h->max_range = maxr;
+    h->max_rxn = maxrx;
+    double mr = std::max(maxr, maxrx);
+    double cell = g.cell_size > 0 ? g.cell_size : (mr > 0 ? 1.01 * mr : g.max_cutoff);
+    P.cell = (float)cell;
+    h->cell = cell;
+    long nc = 1;
+    for (int d = 0; d < 3; ++d) {
+        int l = (int)(P.Lb[d] / P.cell);
+        P.L[d] = l < 1 ? 1 : l;
+        nc *= P.L[d];
+    }
+    if (nc > (1L << 28)) return set_err(h, RD_ERANGE, "too many cells; raise cell_size");
+    h->ncell = (int)nc;
+    P.S = std::max(1, (int)std::ceil(maxr * 1.0001 / cell));
+    P.n_pot2 = (int)h->pot2.size();
+    for (int p = 0; p < P.n_pot2; ++p) {
+        const rd_pot2& q = h->pot2[p];
+        RdPot2Dev& d = P.pot2[p];
+        d.kind = q.type; d.subtype = q.subtype; d.k = q.force_const;
+        d.length = q.length; d.depth = q.depth; d.cutoff = (float)g.max_cutoff;
+        d.wc = q.type == RD_POT2_PIECEWISE_WELL ? (float)(q.depth / (0.25 * q.length * q.length)) : 0.0f;
+    }
+    for (int t = 0; t < nt; ++t) {
+        P.a[t] = (float)(g.dt * h->D[t] / (g.boltzmann * g.temperature));
+        P.b[t] = (float)std::sqrt(2.0 * h->D[t] * g.dt);
+        P.prad[t] = (float)h->rad[t];
+    }
+    P.seed = g.seed;
+    P.stream = g.stream;
+    P.n_pot1 = (int)h->pot1.size();
+    h->pot1dev.resize(h->pot1.size());
+    for (size_t k = 0; k < h->pot1.size(); ++k) {
+        const rd_pot1& q = h->pot1[k];
+        RdPot1Dev& d = h->pot1dev[k];
+        memset(&d, 0, sizeof(d));
+        d.kind = q.type; d.subtype = q.subtype; d.k = q.force_const;
+        double nn = std::sqrt((double)q.normal[0] * q.normal[0] + (double)q.normal[1] * q.normal[1] +
+                              (double)q.normal[2] * q.normal[2]);
+        for (int a = 0; a < 3; ++a) {
+            d.o[a] = q.origin[a];
+            d.ext[a] = q.extension[a];
+            d.n[a] = nn > 0 ? (float)(q.normal[a] / nn) : (a == 2 ? 1.0f : 0.0f);
+        }
+        d.hh = 0.5f * q.height;
+        d.R = q.radius;
+        d.mask = q.type_mask;
+    }
+    return RD_OK;
+}
+
+extern "C" int rd_set_particles(rd_system* h, int n, const float* xyz, const int* type)
+{
+    if (!h) return RD_EINVAL;
+    if (n < 0 || n > RD_MAX_PARTICLES || (n > 0 && (!xyz || !type))) return set_err(h, RD_EINVAL, "bad particle arrays");
+    if (h->nt == 0) return set_err(h, RD_ESTATE, "set particle types first");
+    for (int i = 0; i < n; ++i) {
+        if (type[i] < 0 || type[i] >= h->nt) return set_err(h, RD_EINVAL, "particle type out of range");
+        for (int d = 0; d < 3; ++d) {
+            float x = xyz[3 * i + d];
+            if (!(x >= (float)h->prm.lattice_bounds[2 * d] && x <= (float)h->prm.lattice_bounds[2 * d + 1]))
+                return set_err(h, RD_ERANGE, "particle is out of the box: " + std::to_string(i));
+        }
+    }
+    h->n = n;
+    h->hxyz.assign(xyz, xyz + 3 * (size_t)n);
+    h->htype.assign(type, type + n);
+    h->particles_dirty = true;
+    return RD_OK;
+}
+
+static int ensure_capacity(rd_system* h)
+{
+    int n = std::max(h->n, 1);
+    if (n > h->cap) {
+        drop_graph(h);
+        int cap = std::max(n, h->cap + h->cap / 2);
+        void* ps[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_xyz_io, h->d_type_io };
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        HIPCK(h, hipMalloc(&h->d_ps, sizeof(float4) * cap));
+        HIPCK(h, hipMalloc(&h->d_pn, sizeof(float4) * cap));
+        HIPCK(h, hipMalloc(&h->d_key, sizeof(uint32_t) * cap));
+        HIPCK(h, hipMalloc(&h->d_rank, sizeof(uint32_t) * cap));
+        HIPCK(h, hipMalloc(&h->d_xyz_io, sizeof(float) * 3 * cap));
+        HIPCK(h, hipMalloc(&h->d_type_io, sizeof(int) * cap));
+        h->cap = cap;
+    }
+    if (h->ncell != h->ncell_alloc) {
+        drop_graph(h);
+        if (h->d_count) (void)hipFree(h->d_count);
+        if (h->d_start) (void)hipFree(h->d_start);
+        if (h->d_bsum) (void)hipFree(h->d_bsum);
+        size_t nc = (size_t)h->ncell;
+        HIPCK(h, hipMalloc(&h->d_count, sizeof(uint32_t) * (nc + 16)));
+        HIPCK(h, hipMalloc(&h->d_start, sizeof(uint32_t) * (nc + 16)));
+        h->nscan = nblk((long)nc, RD_SCAN_TILE);
+        HIPCK(h, hipMalloc(&h->d_bsum, sizeof(uint32_t) * (h->nscan + 16)));
+        HIPCK(h, hipMemsetAsync(h->d_count, 0, sizeof(uint32_t) * (nc + 16), h->stream));
+        h->ncell_alloc = h->ncell;
+    }
+    if (!h->d_P) HIPCK(h, hipMalloc(&h->d_P, sizeof(RdDev)));
+    if (!h->d_st) {
+        HIPCK(h, hipMalloc(&h->d_st, sizeof(RdStepState)));
+        RdStepState s0;
+        memset(&s0, 0, sizeof(s0));
+        s0.step = h->step;
+        HIPCK(h, hipMemcpyAsync(h->d_st, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream));
+    }
+    if (!h->d_etot) HIPCK(h, hipMalloc(&h->d_etot, sizeof(double)));
+    return RD_OK;
+}
+
+// scan of count -> start, then scatter pn -> ps
+static void launch_sort(rd_system* h, int advance)
+{
+    int nc = h->ncell;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, h->stream, h->d_bsum, h->nscan, h->d_start, nc);
+    hipLaunchKernelGGL(k_scan_apply, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum,
+                       h->d_start);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(h->n, 1), RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_pn,
+                       h->d_key, h->d_rank, h->d_start, h->d_ps, h->n, h->d_st, advance);
+}
+
+void rd_host_launch_force(rd_system* h);
+
+static void launch_step(rd_system* h)
+{
+    rd_host_launch_force(h);
+    launch_sort(h, 1);
+}
+
+// upload tables and particles, build the cell-sorted arrays
+static int upload(rd_system* h)
+{
+    int rc = build_tables(h);
+    if (rc) return rc;
+    rc = ensure_capacity(h);
+    if (rc) return rc;
+    HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
+    if (h->noise_n >= 0 && h->noise_n != h->n) {
+        // the noise array was laid out for another particle count: back to the generator
+        RdStepState st;
+        HIPCK(h, hipMemcpyAsync(&st, h->d_st, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        st.noise_nsteps = 0;
+        HIPCK(h, hipMemcpyAsync(h->d_st, &st, sizeof(st), hipMemcpyHostToDevice, h->stream));
+        h->noise_n = -1;
+    }
+    if (h->d_pot1) { (void)hipFree(h->d_pot1); h->d_pot1 = nullptr; }
+    if (!h->pot1dev.empty()) {
+        HIPCK(h, hipMalloc(&h->d_pot1, sizeof(RdPot1Dev) * h->pot1dev.size()));
+        HIPCK(h, hipMemcpyAsync(h->d_pot1, h->pot1dev.data(), sizeof(RdPot1Dev) * h->pot1dev.size(),
+                                hipMemcpyHostToDevice, h->stream));
+    }
+    if (h->n > 0) {
+        HIPCK(h, hipMemcpyAsync(h->d_xyz_io, h->hxyz.data(), sizeof(float) * 3 * h->n, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->d_type_io, h->htype.data(), sizeof(int) * h->n, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io, h->d_type_io,
+                           h->n, h->d_P, h->d_pn, h->d_key, h->d_rank, h->d_count);
+    }
+    launch_sort(h, 0);
+    HIPCK(h, hipGetLastError());
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    h->dirty = false;
+    h->particles_dirty = false;
+    drop_graph(h);
+    return RD_OK;
+}
+
+extern "C" int rd_initialize(rd_system* h)
+{
+    if (!h) return RD_EINVAL;
+    if (h->nt == 0) return set_err(h, RD_ESTATE, "no particle types");
+    HIPCK(h, hipSetDevice(h->dev));
+    int rc = upload(h);
+    if (rc) return rc;
+    h->initialized = true;
+    return RD_OK;
+}
+
+static int refresh(rd_system* h)
+{
+    if (!h->initialized) return set_err(h, RD_ESTATE, "rd_initialize has not been called");
+    HIPCK(h, hipSetDevice(h->dev));
+    if (h->dirty || h->particles_dirty) {
+        int rc = rd_copy_pos_from_device(h, nullptr, nullptr);   // keep moved positions unless replaced
+        (void)rc;
+        return upload(h);
+    }
+    return RD_OK;
+}
+
+extern "C" int rd_simulate(rd_system* h, int nsteps)
+{
+    if (!h || nsteps < 0) return RD_EINVAL;
+    int rc = refresh(h);
+    if (rc) return rc;
+    if (nsteps == 0) return RD_OK;
+    if (!h->graph_exec) {
+        HIPCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        launch_step(h);
+        hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
+        if (e != hipSuccess) return set_err(h, RD_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        HIPCK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < nsteps; ++i) HIPCK(h, hipGraphLaunch(h->graph_exec, h->stream));
+    h->step += (uint64_t)nsteps;
+    RdStepState st;
+    HIPCK(h, hipMemcpyAsync(&st, h->d_st, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    if (st.err & 1u) return set_err(h, RD_ERANGE, "non-finite particle position");
+    if (st.err & 2u) return set_err(h, RD_ERANGE, "noise array exhausted");
+    return RD_OK;
+}
+
+// pieces of the step for the measurement entry points (rd_bench.hip)
+int rd_host_refresh(rd_system* h) { return refresh(h); }
+void rd_host_launch_sort(rd_system* h, int advance) { launch_sort(h, advance); }
+void rd_host_launch_force(rd_system* h)
+{
+    if (h->n > 0)
+        hipLaunchKernelGGL(k_force_update, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_pn,
+                           h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P, h->d_pot1, h->d_st, h->d_noise,
+                           h->n);
+}
+
+extern "C" int rd_copy_pos_from_device(rd_system* h, float* xyz, int* type)
+{
+    if (!h) return RD_EINVAL;
+    if (!h->initialized) return set_err(h, RD_ESTATE, "not initialized");
+    if (h->particles_dirty) {
+        if (xyz) memcpy(xyz, h->hxyz.data(), sizeof(float) * 3 * h->n);
+        if (type) memcpy(type, h->htype.data(), sizeof(int) * h->n);
+        return RD_OK;
+    }
+    HIPCK(h, hipSetDevice(h->dev));
+    if (h->n > 0) {
+        hipLaunchKernelGGL(k_unsort, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->n,
+                           h->d_xyz_io, h->d_type_io);
+        HIPCK(h, hipMemcpyAsync(h->hxyz.data(), h->d_xyz_io, sizeof(float) * 3 * h->n, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->htype.data(), h->d_type_io, sizeof(int) * h->n, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    if (xyz) memcpy(xyz, h->hxyz.data(), sizeof(float) * 3 * h->n);
+    if (type) memcpy(type, h->htype.data(), sizeof(int) * h->n);
+    return RD_OK;
+}
+
+extern "C" int rd_cell_ids(rd_system* h, int* field)
+{
+    if (!h || !field) return RD_EINVAL;
+    int rc = refresh(h);
+    if (rc) return rc;
+    if (h->n == 0) return RD_OK;
+    hipLaunchKernelGGL(k_cell_ids, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->n, h->d_P,
+                       h->d_type_io);
+    HIPCK(h, hipGetLastError());
+    HIPCK(h, hipMemcpyAsync(field, h->d_type_io, sizeof(int) * h->n, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    return RD_OK;
+}
+
+extern "C" int rd_gradients(rd_system* h, float* grad, double* energy)
+{
+    if (!h) return RD_EINVAL;
+    int rc = refresh(h);
+    if (rc) return rc;
+    if (h->n == 0) { if (energy) *energy = 0.0; return RD_OK; }
+    HIPCK(h, hipMemsetAsync(h->d_etot, 0, sizeof(double), h->stream));
+    hipLaunchKernelGGL(k_gradients, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
+                       h->d_P, h->d_pot1, h->n, h->d_xyz_io, h->d_etot);
+    HIPCK(h, hipGetLastError());
+    std::vector<float> tmp(3 * (size_t)h->n);
+    double e = 0.0;
+    HIPCK(h, hipMemcpyAsync(tmp.data(), h->d_xyz_io, sizeof(float) * 3 * h->n, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipMemcpyAsync(&e, h->d_etot, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    if (grad) memcpy(grad, tmp.data(), sizeof(float) * 3 * h->n);
+    if (energy) *energy = e;
+    return RD_OK;
+}
+
+extern "C" int rd_noise_fill(rd_system* h, uint64_t step0, int nsteps, float* out)
+{
+    if (!h || nsteps < 0 || !out) return RD_EINVAL;
+    int rc = refresh(h);
+    if (rc) return rc;
+    size_t total = (size_t)nsteps * (size_t)h->n;
+    if (total == 0) return RD_OK;
+    float* d = nullptr;
+    HIPCK(h, hipMalloc(&d, sizeof(float) * 3 * total));
+    hipLaunchKernelGGL(k_noise_fill, dim3((unsigned)((total + RD_BLOCK - 1) / RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream,
+                       h->d_P, (unsigned long long)step0, nsteps, h->n, d);
+    hipError_t e1 = hipMemcpyAsync(out, d, sizeof(float) * 3 * total, hipMemcpyDeviceToHost, h->stream);
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e1 != hipSuccess || e2 != hipSuccess) return set_err(h, RD_EHIP, "noise fill failed");
+    return RD_OK;
+}
+
+extern "C" int rd_set_noise_array(rd_system* h, const float* noise, int nsteps)
+{
+    if (!h || nsteps < 0 || (nsteps > 0 && !noise)) return RD_EINVAL;
+    int rc = refresh(h);
+    if (rc) return rc;
+    RdStepState st;
+    HIPCK(h, hipMemcpyAsync(&st, h->d_st, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    size_t total = (size_t)nsteps * 3 * (size_t)h->n;
+    if (total > h->noise_cap) {
+        drop_graph(h);
+        if (h->d_noise) (void)hipFree(h->d_noise);
+        h->d_noise = nullptr;
+        HIPCK(h, hipMalloc(&h->d_noise, sizeof(float) * total));
+        h->noise_cap = total;
+    }
+    if (total) HIPCK(h, hipMemcpyAsync(h->d_noise, noise, sizeof(float) * total, hipMemcpyHostToDevice, h->stream));
+    st.noise_step0 = st.step;
+    st.noise_nsteps = nsteps;
+    h->noise_n = nsteps > 0 ? h->n : -1;
+    st.err &= ~2u;
+    HIPCK(h, hipMemcpyAsync(h->d_st, &st, sizeof(st), hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    drop_graph(h);   // the captured kernel holds the noise pointer
+    return RD_OK;
+}
+
+// ---- RDF ----
+extern "C" int rd_rdf_configure(rd_system* h, double rmax, int nbins)
+{
+    if (!h || !(rmax > 0) || nbins < 1) return RD_EINVAL;
+    if (h->nt == 0) return set_err(h, RD_ESTATE, "set particle types first");
+    for (int d = 0; d < 3; ++d)
+        if (rmax > 0.5 * (h->prm.lattice_bounds[2 * d + 1] - h->prm.lattice_bounds[2 * d]))
+            return set_err(h, RD_EINVAL, "RDF range exceeds half the box");
+    h->rdf_rmax = rmax;
+    h->rdf_bins = nbins;
+    size_t nh = (size_t)h->nt * h->nt * nbins;
+    if (h->d_rdf) (void)hipFree(h->d_rdf);
+    h->d_rdf = nullptr;
+    HIPCK(h, hipMalloc(&h->d_rdf, sizeof(unsigned long long) * nh));
+    HIPCK(h, hipMemsetAsync(h->d_rdf, 0, sizeof(unsigned long long) * nh, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    h->rdf_frames = 0;
+    h->rdf_npt_sum.assign(h->nt, 0.0);
+    h->rdf_pairs_sum.assign((size_t)h->nt * h->nt, 0.0);
+    h->rdf_vol = 1.0;
+    for (int d = 0; d < 3; ++d) h->rdf_vol *= h->prm.lattice_bounds[2 * d + 1] - h->prm.lattice_bounds[2 * d];
+    return RD_OK;
+}
+
+extern "C" int rd_rdf_reset(rd_system* h)
+{
+    if (!h || !h->d_rdf) return RD_ESTATE;
+    return rd_rdf_configure(h, h->rdf_rmax, h->rdf_bins);
+}
+
+extern "C" int rd_rdf_calculate(rd_system* h)
+{
+    if (!h) return RD_EINVAL;
+    if (!h->d_rdf) return set_err(h, RD_ESTATE, "rd_rdf_configure first");
+    int rc = refresh(h);
+    if (rc) return rc;
+    int S = std::max(1, (int)std::ceil(h->rdf_rmax * 1.0001 / h->cell));
+    size_t nh = (size_t)h->nt * h->nt * h->rdf_bins;
+    size_t lds = nh * sizeof(uint32_t);
+    if (h->n > 0) {
+        if (lds <= 160 * 1024) {
+            if (lds > 64 * 1024)
+                HIPCK(h, hipFuncSetAttribute((const void*)k_rdf<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(k_rdf<true>, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), lds, h->stream, h->d_ps,
+                               h->d_start, h->d_P, S, (float)h->rdf_rmax, h->rdf_bins, h->n, h->d_rdf);
+        } else
+            hipLaunchKernelGGL(k_rdf<false>, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps,
+                               h->d_start, h->d_P, S, (float)h->rdf_rmax, h->rdf_bins, h->n, h->d_rdf);
+        HIPCK(h, hipGetLastError());
+    }
+    // per-frame normalisation constants (particle numbers may change by reactions)
+    std::vector<long> npt(h->nt, 0);
+    for (int i = 0; i < h->n; ++i) npt[h->htype[i]]++;
+    for (int a = 0; a < h->nt; ++a) {
+        h->rdf_npt_sum[a] += (double)npt[a];
+        for (int b = 0; b < h->nt; ++b)
+            h->rdf_pairs_sum[(size_t)a * h->nt + b] += (double)npt[a] * (double)(npt[b] - (a == b ? 1 : 0));
+    }
+    h->rdf_frames++;
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    return RD_OK;
+}
+
+extern "C" int rd_rdf_copy(rd_system* h, uint64_t* counts, long* frames)
+{
+    if (!h || !counts) return RD_EINVAL;
+    if (!h->d_rdf) return set_err(h, RD_ESTATE, "rd_rdf_configure first");
+    size_t nh = (size_t)h->nt * h->nt * h->rdf_bins;
+    HIPCK(h, hipMemcpyAsync(counts, h->d_rdf, sizeof(uint64_t) * nh, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    if (frames) *frames = h->rdf_frames;
+    return RD_OK;
+}
+
+// g_ab(r_k) = C_ab(k) / (sum_frames N_a (N_b - [a==b]) / V * V_shell(k)), FP64.
+extern "C" int rd_rdf_normalize(rd_system* h, double* g)
+{
+    if (!h || !g) return RD_EINVAL;
+    if (!h->d_rdf) return set_err(h, RD_ESTATE, "rd_rdf_configure first");
+    int nt = h->nt, nb = h->rdf_bins;
+    std::vector<uint64_t> c((size_t)nt * nt * nb);
+    int rc = rd_rdf_copy(h, c.data(), nullptr);
+    if (rc) return rc;
+    const double pi = 3.14159265358979323846;
+    double dr = h->rdf_rmax / nb;
+    for (int a = 0; a < nt; ++a)
+        for (int b = 0; b < nt; ++b)
+            for (int k = 0; k < nb; ++k) {
+                double r1 = k * dr, r2 = (k + 1) * dr;
+                double vs = 4.0 / 3.0 * pi * (r2 * r2 * r2 - r1 * r1 * r1);
+                double den = h->rdf_pairs_sum[(size_t)a * nt + b] / h->rdf_vol * vs;
+                size_t idx = ((size_t)a * nt + b) * nb + k;
+                g[idx] = den > 0 ? (double)c[idx] / den : 0.0;
+            }
+    return RD_OK;
+}
+
+// ---- pair sets ----
+static int pairs_common(rd_system* h, bool reactions, int64_t* out, long cap, long* count)
+{
+    if (!h || cap < 0 || (cap > 0 && !out)) return RD_EINVAL;
+    int rc = refresh(h);
+    if (rc) return rc;
+    const float* table = reactions ? h->P.rr2 : h->P.rc2;
+    const float* dtable = (const float*)((const char*)h->d_P + (reactions ? offsetof(RdDev, rr2) : offsetof(RdDev, rc2)));
+    float m2 = 0.0f;
+    for (int a = 0; a < h->nt * h->nt; ++a) m2 = std::max(m2, table[a]);
+    if (h->n == 0 || m2 <= 0.0f) { if (count) *count = 0; return RD_OK; }
+    int S = std::max(1, (int)std::ceil(std::sqrt((double)m2) * 1.0001 / h->cell));
+    if (!h->d_pcount) HIPCK(h, hipMalloc(&h->d_pcount, sizeof(unsigned long long)));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (h->pairs_cap == 0) {
+            if (h->d_pairs) (void)hipFree(h->d_pairs);
+            h->d_pairs = nullptr;
+            size_t c0 = std::max<size_t>(1 << 16, (size_t)h->n * 4);
+            HIPCK(h, hipMalloc(&h->d_pairs, sizeof(uint2) * c0));
+            h->pairs_cap = c0;
+        }
+        HIPCK(h, hipMemsetAsync(h->d_pcount, 0, sizeof(unsigned long long), h->stream));
+        hipLaunchKernelGGL(k_pairs, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
+                           h->d_P, dtable, S, h->n, h->d_pairs, (unsigned long long)h->pairs_cap, h->d_pcount);
+        HIPCK(h, hipGetLastError());
+        unsigned long long total = 0;
+        HIPCK(h, hipMemcpyAsync(&total, h->d_pcount, sizeof(total), hipMemcpyDeviceToHost, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        if (total > h->pairs_cap) {
+            (void)hipFree(h->d_pairs);
+            h->d_pairs = nullptr;
+            HIPCK(h, hipMalloc(&h->d_pairs, sizeof(uint2) * total));
+            h->pairs_cap = total;
+            continue;
+        }
+        std::vector<uint64_t> v(total);
+        std::vector<uint2> raw(total);
+        if (total) HIPCK(h, hipMemcpy(raw.data(), h->d_pairs, sizeof(uint2) * total, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < total; ++k) v[k] = ((uint64_t)raw[k].x << 32) | raw[k].y;
+        std::sort(v.begin(), v.end());
+        long nout = (long)std::min<uint64_t>(total, (uint64_t)cap);
+        for (long k = 0; k < nout; ++k) { out[2 * k] = (int64_t)(v[k] >> 32); out[2 * k + 1] = (int64_t)(v[k] & 0xffffffffu); }
+        if (count) *count = (long)total;
+        return RD_OK;
+    }
+    return set_err(h, RD_ENOMEM, "pair buffer");
+}
+
+extern "C" int rd_neighbor_pairs(rd_system* h, int64_t* out, long cap, long* count)
+{
+    return pairs_common(h, false, out, cap, count);
+}
+
+extern "C" int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long* count)
+{
+    return pairs_common(h, true, out, cap, count);
+}
+
+// ---- reactions: lib/ReaDDyGPU.cpp:303-463 ----
+extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
+{
+    if (!h || !cr) return RD_EINVAL;
+    if (!h->initialized) return set_err(h, RD_ESTATE, "not initialized");
+    int nr = (int)cr[0];
+    if (nr <= 0) return RD_OK;
+    int rc = rd_copy_pos_from_device(h, nullptr, nullptr);
+    if (rc) return rc;
+    std::vector<float> nx(h->hxyz);
+    std::vector<int> ntp(h->htype);
+    for (int r = 0; r < nr; ++r) {
+        const float* e = &cr[1 + 6 * r];
+        int newType = (int)e[1];
+        int index = (int)e[5];
+        if (index != -1) {
+            if (index < 0 || index >= h->n || ntp[index] == -1)
+                return set_err(h, RD_EINVAL, "reaction deletes a nonexistent particle");
+            ntp[index] = -1;
+        }
+        if (newType != -1) {
+            if (newType < 0 || newType >= h->nt) return set_err(h, RD_EINVAL, "reaction creates an unknown type");
+            nx.push_back(e[2]); nx.push_back(e[3]); nx.push_back(e[4]);
+            ntp.push_back(newType);
+        }
+    }
+    std::vector<float> cx;
+    std::vector<int> ct;
+    for (size_t i = 0; i < ntp.size(); ++i)
+        if (ntp[i] != -1) {
+            cx.push_back(nx[3 * i]); cx.push_back(nx[3 * i + 1]); cx.push_back(nx[3 * i + 2]);
+            ct.push_back(ntp[i]);
+        }
+    rc = rd_set_particles(h, (int)ct.size(), cx.data(), ct.data());
+    if (rc) return rc;
+    return refresh(h);
+}
+
+extern "C" int rd_get_info(rd_system* h, rd_info* info)
+{
+    if (!h || !info) return RD_EINVAL;
+    memset(info, 0, sizeof(*info));
+    info->n = h->n;
+    info->n_types = h->nt;
+    for (int d = 0; d < 3; ++d) info->lattice[d] = h->P.L[d];
+    info->cell_size = h->cell;
+    info->stencil = h->P.S;
+    info->step = h->step;
+    info->rdf_frames = h->rdf_frames;
+    return RD_OK;
+}

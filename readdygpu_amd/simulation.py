@@ -1,0 +1,205 @@
+"""Host-side mirror of the reference's Simulation / CudaSimulation interface.
+
+``Simulation`` drives libCReaDDyGPU.so through its C ABI.  Method names follow
+the reference's C++ methods (lib/ReaDDyGPU_Cpp/Simulation.hpp:161-170,
+CudaKernels/CudaSimulation.cuh:40-71) in Python spelling; arguments are
+numpy arrays in particle-index order.  Every call raises ``RdError`` on a
+nonzero status, as the reference's methods return 1 and print.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .model import SystemSpec
+
+
+class RdError(RuntimeError):
+    pass
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Simulation:
+    """One periodic BD system on one GPU (Simulation + CudaSimulation)."""
+
+    def __init__(self, spec: SystemSpec, xyz: np.ndarray, types: np.ndarray, device: int = 0,
+                 rdf: Optional[tuple] = None, initialize: bool = True):
+        self.lib = _lib.load()
+        self.spec = spec
+        prm = _lib.rd_params()
+        for d in range(3):
+            prm.lattice_bounds[2 * d] = spec.box_lo[d]
+            prm.lattice_bounds[2 * d + 1] = spec.box_hi[d]
+        prm.dt = spec.dt
+        prm.boltzmann = spec.kB
+        prm.temperature = spec.T
+        prm.max_cutoff = spec.max_cutoff
+        prm.cell_size = spec.cell_size
+        prm.seed = spec.seed
+        prm.stream = spec.stream
+        prm.device = device
+        h = C.c_void_p()
+        rc = self.lib.rd_create(C.byref(prm), C.byref(h))
+        if rc != 0:
+            raise RdError(f"rd_create failed ({rc}): no HIP device or bad parameters")
+        self.h = h
+        nt = spec.n_types
+        D = np.array([t.D for t in spec.types], dtype=np.float64)
+        rad = np.array([t.radius for t in spec.types], dtype=np.float64)
+        radii = np.zeros((nt, nt), dtype=np.float64)
+        for i, t in enumerate(spec.types):
+            if t.collision_radii is not None:
+                radii[i] = np.asarray(t.collision_radii, dtype=np.float64)
+        self._check(self.lib.rd_set_particle_types(h, nt, _ptr(D), _ptr(rad), _ptr(radii)))
+        for p in spec.pot1:
+            q = _lib.rd_pot1()
+            q.type, q.subtype, q.force_const = p.kind, p.subtype, p.k
+            for d in range(3):
+                q.origin[d] = p.origin[d]
+                q.normal[d] = p.normal[d]
+                q.extension[d] = p.extension[d]
+            q.height, q.radius = p.height, p.radius
+            q.type_mask = spec.pot1_mask(p)
+            self._check(self.lib.rd_add_order_one_potential(h, C.byref(q)))
+        for p in spec.pot2:
+            q = _lib.rd_pot2()
+            q.type, q.subtype, q.force_const, q.length, q.depth = p.kind, p.subtype, p.k, p.length, p.depth
+            m = np.ascontiguousarray(spec.pot2_mask(p), dtype=np.uint8)
+            self._check(self.lib.rd_add_order_two_potential(h, C.byref(q), _ptr(m)))
+        if spec.reaction_radii is not None:
+            R = np.ascontiguousarray(spec.reaction_radii, dtype=np.float64)
+            self._check(self.lib.rd_set_reaction_radii(h, _ptr(R)))
+        self.set_particles(xyz, types)
+        if rdf is not None:
+            self.configure_rdf(*rdf)
+        if initialize:
+            self.initialize()
+
+    # -- plumbing --
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self.lib.rd_last_error(self.h)
+            raise RdError(f"rd_* returned {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- reference interface --
+    def set_particles(self, xyz: np.ndarray, types: np.ndarray):
+        """copyPosToDevice + copyTypesToDevice (CudaSimulation.cu:905-970)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        types = np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
+        if len(xyz) != len(types):
+            raise ValueError("xyz and types differ in length")
+        self._check(self.lib.rd_set_particles(self.h, len(types), _ptr(xyz), _ptr(types)))
+
+    def initialize(self):
+        """CudaSimulation::initialize (CudaSimulation.cu:138-455)."""
+        self._check(self.lib.rd_initialize(self.h))
+
+    def simulate(self, nsteps: int = 1):
+        """Simulation::simulate(int) (Simulation.cpp:349-355)."""
+        self._check(self.lib.rd_simulate(self.h, int(nsteps)))
+
+    def copy_pos_from_device(self):
+        """copyPosFromDevice (CudaSimulation.cu:972-984) -> (xyz [N,3] f32, types [N])."""
+        n = self.info().n
+        xyz = np.empty((n, 3), dtype=np.float32)
+        t = np.empty(n, dtype=np.int32)
+        self._check(self.lib.rd_copy_pos_from_device(self.h, _ptr(xyz), _ptr(t)))
+        return xyz, t
+
+    def cell_ids(self) -> np.ndarray:
+        """Reference cell index (CudaSimulation.cu:586-588) per particle."""
+        n = self.info().n
+        f = np.empty(n, dtype=np.int32)
+        self._check(self.lib.rd_cell_ids(self.h, _ptr(f)))
+        return f
+
+    def gradients(self):
+        """grad U [N,3] (f32) and total energy (FP64 accumulation)."""
+        n = self.info().n
+        g = np.empty((n, 3), dtype=np.float32)
+        e = C.c_double()
+        self._check(self.lib.rd_gradients(self.h, _ptr(g), C.byref(e)))
+        return g, e.value
+
+    def noise(self, step0: int, nsteps: int) -> np.ndarray:
+        """The FP32 N(0,1) array the step uses: [nsteps, 3, N]."""
+        n = self.info().n
+        out = np.empty((nsteps, 3, n), dtype=np.float32)
+        self._check(self.lib.rd_noise_fill(self.h, int(step0), int(nsteps), _ptr(out)))
+        return out
+
+    def set_noise_array(self, noise: Optional[np.ndarray]):
+        """Take the given noise [nsteps, 3, N] for the next nsteps steps (None: generator)."""
+        if noise is None:
+            self._check(self.lib.rd_set_noise_array(self.h, None, 0))
+            return
+        noise = np.ascontiguousarray(noise, dtype=np.float32)
+        self._noise_keep = noise
+        self._check(self.lib.rd_set_noise_array(self.h, _ptr(noise), noise.shape[0]))
+
+    def configure_rdf(self, r_max: float, n_bins: int):
+        self._check(self.lib.rd_rdf_configure(self.h, float(r_max), int(n_bins)))
+        self._rdf_shape = (self.spec.n_types, self.spec.n_types, int(n_bins))
+
+    def call_rdf_calculation(self):
+        """callRDFCalculation (CudaSimulation.cu:505-520)."""
+        self._check(self.lib.rd_rdf_calculate(self.h))
+
+    def copy_rdf_matrix(self):
+        """copyRDFMatrix (CudaSimulation.cu:457-481): raw ordered-pair counts."""
+        c = np.zeros(self._rdf_shape, dtype=np.uint64)
+        frames = C.c_long()
+        self._check(self.lib.rd_rdf_copy(self.h, _ptr(c), C.byref(frames)))
+        return c, frames.value
+
+    def normalize_rdf(self) -> np.ndarray:
+        """normalizeRDF (Simulation.cpp:40-62), FP64 with shell and box volume."""
+        g = np.zeros(self._rdf_shape, dtype=np.float64)
+        self._check(self.lib.rd_rdf_normalize(self.h, _ptr(g)))
+        return g
+
+    def _pairs(self, fn) -> np.ndarray:
+        cnt = C.c_long()
+        self._check(fn(self.h, None, 0, C.byref(cnt)))
+        out = np.empty((max(cnt.value, 1), 2), dtype=np.int64)
+        self._check(fn(self.h, _ptr(out), cnt.value, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def neighbor_pairs(self) -> np.ndarray:
+        return self._pairs(self.lib.rd_neighbor_pairs)
+
+    def reaction_candidates(self) -> np.ndarray:
+        return self._pairs(self.lib.rd_reaction_candidates)
+
+    def apply_reactions(self, creactions: np.ndarray):
+        """cReactions protocol (lib/ReaDDyGPU.cpp:303-463)."""
+        cr = np.ascontiguousarray(creactions, dtype=np.float32)
+        self._check(self.lib.rd_apply_reactions(self.h, _ptr(cr)))
+
+    def info(self) -> _lib.rd_info:
+        i = _lib.rd_info()
+        self._check(self.lib.rd_get_info(self.h, C.byref(i)))
+        return i

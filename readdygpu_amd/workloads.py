@@ -1,0 +1,68 @@
+"""Seeded synthetic workloads.
+
+plain_10k   10,000 particles, 2 species, 100 nm periodic box, harmonic
+            repulsion k=10, dt=0.1 ns, 1,000 steps, RDF 0-20 nm / 200 bins.
+plain_1m    1,000,000 particles, 4 species, 1000 nm box, repulsion + a
+            piecewise harmonic well (depth 1, width 1) between species 0 and 1,
+            spherical confinement R=480 nm, A+B candidates at 4 nm, dt=0.05 ns,
+            500 steps, RDF 0-20 nm / 200 bins.
+
+Values the workload descriptions leave open are fixed here: T = 293.15 K with
+kB = 0.0083144621 kJ/mol/K (the reference GPU input's constants), sphere
+k = 1 kJ/mol/nm^2, well inner stiffness 0 (the harmonic repulsion supplies it),
+maxCutoff = 10 nm as in lib/ReaDDyGPU.cpp:101.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .model import (ATTRACTIVE, HARMONIC, PIECEWISE_WELL, REPULSIVE, SPHERE, OrderOnePotential,
+                    OrderTwoPotential, ParticleType, SystemSpec)
+
+
+def uniform_positions(rng: np.random.Generator, n: int, lo, hi) -> np.ndarray:
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    x = (lo + rng.random((n, 3)) * (hi - lo)).astype(np.float32)
+    # keep float32 rounding inside [lo, hi)
+    return np.minimum(np.maximum(x, lo.astype(np.float32)), np.nextafter(hi.astype(np.float32), -np.inf))
+
+
+def plain_10k(n: int = 10_000, seed: int = 1, cell_size: float = 0.0):
+    spec = SystemSpec(box_lo=(0.0, 0.0, 0.0), box_hi=(100.0, 100.0, 100.0), dt=0.1, seed=seed,
+                      cell_size=cell_size)
+    spec.types = [ParticleType("A", D=1.0, radius=1.5), ParticleType("B", D=0.5, radius=2.0)]
+    spec.pot2 = [OrderTwoPotential(HARMONIC, REPULSIVE, k=10.0)]
+    rng = np.random.default_rng(seed)
+    xyz = uniform_positions(rng, n, spec.box_lo, spec.box_hi)
+    types = np.repeat(np.arange(2, dtype=np.int32), [n // 2, n - n // 2])
+    return spec, xyz, types
+
+
+PLAIN_10K = dict(steps=1000, rdf_every=100, rdf_range=20.0, rdf_bins=200)
+
+
+def plain_1m(n: int = 1_000_000, seed: int = 2, cell_size: float = 0.0, box: float = 1000.0):
+    spec = SystemSpec(box_lo=(0.0, 0.0, 0.0), box_hi=(box, box, box), dt=0.05, seed=seed,
+                      cell_size=cell_size)
+    radii = np.linspace(1.0, 3.0, 4)
+    Ds = np.linspace(0.1, 2.0, 4)
+    spec.types = [ParticleType(f"S{i}", D=float(Ds[i]), radius=float(radii[i])) for i in range(4)]
+    spec.pot2 = [
+        OrderTwoPotential(HARMONIC, REPULSIVE, k=10.0),
+        OrderTwoPotential(PIECEWISE_WELL, ATTRACTIVE, k=0.0, length=1.0, depth=1.0, pairs=[(0, 1)]),
+    ]
+    c = 0.5 * box
+    spec.pot1 = [OrderOnePotential(SPHERE, ATTRACTIVE, k=1.0, origin=(c, c, c), radius=0.48 * box)]
+    R = np.zeros((4, 4))
+    R[0, 1] = R[1, 0] = 4.0
+    spec.reaction_radii = R
+    rng = np.random.default_rng(seed)
+    xyz = uniform_positions(rng, n, spec.box_lo, spec.box_hi)
+    types = np.repeat(np.arange(4, dtype=np.int32), n // 4)
+    if len(types) < n:
+        types = np.concatenate([types, np.zeros(n - len(types), dtype=np.int32)])
+    return spec, xyz, types
+
+
+PLAIN_1M = dict(steps=500, rdf_range=20.0, rdf_bins=200)
