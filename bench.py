@@ -79,6 +79,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cell", type=float, default=0.0, help="lattice cell edge (0: automatic)")
     ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
+    ap.add_argument("--kernel", type=int, default=-1, help="force kernel: 1 gather, 2 tiled")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--fill", type=float, default=0.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported", file=sys.stderr)
@@ -91,8 +96,14 @@ def main():
     n = len(types)
     log(f"setting up plain_1m n={n}")
     sim = Simulation(spec, xyz, types)
+    if args.kernel >= 0:
+        sim.set_launch_config(kernel=args.kernel, block=args.block, rows_per_tile=args.rows, fill=args.fill)
+    if not args.no_tune and args.kernel < 0 and args.cell == 0.0:
+        ms = sim.autotune(trial_steps=10)
+        log(f"autotuned: {ms * 1e3:.1f} us/step")
     info = sim.info()
-    log(f"lattice {list(info.lattice)} cell {info.cell_size:.3f} nm; warmup {args.warmup} steps")
+    lc = sim.launch_config()
+    log(f"lattice {list(info.lattice)} cell {info.cell_size:.3f} nm; kernel {lc}; warmup {args.warmup} steps")
     sim.simulate(args.warmup)
 
     out = (C.c_double * 4)()
@@ -141,6 +152,9 @@ def main():
             "n_particles": n,
             "cell_size_nm": info.cell_size,
             "lattice": list(info.lattice),
+            "force_kernel": "tiled" if lc["used_kernel"] == 2 else "gather",
+            "block": lc["block"],
+            "rows_per_tile": lc["used_rows"],
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -11,10 +11,11 @@
 extern "C" {
 #endif
 
-/* Run nsteps steps as one captured graph with HIP events around the
- * force/update kernel and around the cell sort of every step.
+/* Run nsteps steps as one captured graph (timed on the host), then nsteps
+ * more steps launched directly with HIP events around the force/update kernel
+ * and the cell sort of every step (graph nodes carry no event timestamps).
  * out[0] = host wall time of graph launch + synchronize (ms),
- * out[1] = first event to last (ms), out[2] = sum of force/update kernel
+ * out[1] = first event to last of pass 2 (ms), out[2] = sum of force/update kernel
  * times, out[3] = sum of cell-sort times. */
 int rd_bench_steps(rd_system* h, int nsteps, double* out);
 

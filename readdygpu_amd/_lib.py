@@ -52,6 +52,20 @@ class rd_pot2(C.Structure):
     ]
 
 
+class rd_launch_config(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int),
+        ("block", C.c_int),
+        ("rows_per_tile", C.c_int),
+        ("fill", C.c_double),
+        ("used_kernel", C.c_int),
+        ("used_rows", C.c_int),
+        ("used_segments", C.c_int),
+        ("used_cap", C.c_int),
+        ("used_lds", C.c_long),
+    ]
+
+
 class rd_info(C.Structure):
     _fields_ = [
         ("n", C.c_int),
@@ -90,6 +104,9 @@ _SIGS = {
     "rd_reaction_candidates": (C.c_int, [P, P, C.c_long, C.POINTER(C.c_long)]),
     "rd_apply_reactions": (C.c_int, [P, P]),
     "rd_get_info": (C.c_int, [P, C.POINTER(rd_info)]),
+    "rd_set_launch_config": (C.c_int, [P, C.POINTER(rd_launch_config)]),
+    "rd_get_launch_config": (C.c_int, [P, C.POINTER(rd_launch_config)]),
+    "rd_autotune": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
     "rd_bench_steps": (C.c_int, [P, C.c_int, P]),
     "rd_bench_copy": (C.c_int, [P, C.c_long, C.c_int, C.POINTER(C.c_double)]),
 }

@@ -175,15 +175,16 @@ __device__ __forceinline__ float3 rd_normal3(uint64_t seed, uint32_t stream, uin
 // WEAK_HARMONIC   CudaSimulation.cu:1021-1031
 // PIECEWISE_WELL  P2_WeakInteractionPiecewiseHarmonic.java:114-200
 // ---------------------------------------------------------------------------
+// rinv = 1/r from v_rsq_f32 (forces need no IEEE division)
 template <bool ENERGY>
-__device__ __forceinline__ float pair_pre(const RdPot2Dev& p, float r0, float r, float* e)
+__device__ __forceinline__ float pair_pre_fast(const RdPot2Dev& p, float r0, float r, float rinv, float* e)
 {
     float pre = 0.0f;
     if (r > p.cutoff) return 0.0f;
     if (p.kind == 1) {
         bool on = (p.subtype == 1 && r > r0) || (p.subtype == 2 && r < r0) || (p.subtype == 3);
         if (on) {
-            pre = p.k * (r - r0) / r;
+            pre = p.k * (r - r0) * rinv;
             if (ENERGY) *e += 0.5f * p.k * (r - r0) * (r - r0);
         }
     } else if (p.kind == 2) {
@@ -192,25 +193,38 @@ __device__ __forceinline__ float pair_pre(const RdPot2Dev& p, float r0, float r,
         if (ENERGY && r < ri) {
             float a = r0 + ri, b = r0 * ri;
             float s = r > r0 ? r : r0;
-            float gs = -s * s * s * s * 0.25f + a * s * s * s / 3.0f - b * s * s * 0.5f;
-            float gi = -ri * ri * ri * ri * 0.25f + a * ri * ri * ri / 3.0f - b * ri * ri * 0.5f;
+            float gs = -s * s * s * s * 0.25f + a * s * s * s * 0.333333343f - b * s * s * 0.5f;
+            float gi = -ri * ri * ri * ri * 0.25f + a * ri * ri * ri * 0.333333343f - b * ri * ri * 0.5f;
             *e += -p.k * (gi - gs);
         }
     } else if (p.kind == 3) {
         float l = p.length;
         float hl = 0.5f * l;
         if (r < r0) {
-            pre = p.k * (r - r0) / r;
+            pre = p.k * (r - r0) * rinv;
             if (ENERGY) *e += 0.5f * p.k * (r - r0) * (r - r0) - p.depth;
         } else if (r < r0 + hl) {
-            pre = p.wc * (r - r0) / r;
+            pre = p.wc * (r - r0) * rinv;
             if (ENERGY) *e += 0.5f * p.wc * (r - r0) * (r - r0) - p.depth;
         } else if (r < r0 + l) {
-            pre = -p.wc * (r - r0 - l) / r;
+            pre = -p.wc * (r - r0 - l) * rinv;
             if (ENERGY) *e += -0.5f * p.wc * (r - r0 - l) * (r - r0 - l);
         }
     }
     return pre;
+}
+
+// |v| and 1/|v| from v_rsq_f32; (0, 0) for v = 0
+__device__ __forceinline__ void fast_len(float v2, float* len, float* inv)
+{
+    if (v2 > 0.0f) {
+        float ri = __builtin_amdgcn_rsqf(v2);
+        *inv = ri;
+        *len = v2 * ri;
+    } else {
+        *inv = 0.0f;
+        *len = 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -228,7 +242,8 @@ __device__ __forceinline__ void ext_grad(const RdPot1Dev& p, float prad, float x
         float vx = x - p.o[0], vy = y - p.o[1], vz = z - p.o[2];
         float s = vx * p.n[0] + vy * p.n[1] + vz * p.n[2];
         float qx = vx - s * p.n[0], qy = vy - s * p.n[1], qz = vz - s * p.n[2];
-        float r = sqrtf(qx * qx + qy * qy + qz * qz);
+        float r, rinv;
+        fast_len(qx * qx + qy * qy + qz * qz, &r, &rinv);
         float as = fabsf(s);
         float sg = s > 0.0f ? 1.0f : (s < 0.0f ? -1.0f : 0.0f);
         if (p.subtype == 1) {
@@ -242,7 +257,7 @@ __device__ __forceinline__ void ext_grad(const RdPot1Dev& p, float prad, float x
             }
             gx += gn * p.n[0]; gy += gn * p.n[1]; gz += gn * p.n[2];
             if (r > p.R) {
-                float pre = p.k * (r - p.R) / r;
+                float pre = p.k * (r - p.R) * rinv;
                 gx += pre * qx; gy += pre * qy; gz += pre * qz;
                 if (ENERGY) *e += 0.5f * p.k * (r - p.R) * (r - p.R);
             }
@@ -253,17 +268,18 @@ __device__ __forceinline__ void ext_grad(const RdPot1Dev& p, float prad, float x
                 if (ENERGY) *e += 0.5f * p.k * (as - p.hh) * (as - p.hh);
             }
             if (r < p.R && r > 0.0f) {
-                float pre = p.k * (r - p.R) / r;
+                float pre = p.k * (r - p.R) * rinv;
                 gx += pre * qx; gy += pre * qy; gz += pre * qz;
                 if (ENERGY) *e += 0.5f * p.k * (r - p.R) * (r - p.R);
             }
         }
     } else if (p.kind == 3) {
         float vx = x - p.o[0], vy = y - p.o[1], vz = z - p.o[2];
-        float d = sqrtf(vx * vx + vy * vy + vz * vz);
+        float d, dinv;
+        fast_len(vx * vx + vy * vy + vz * vz, &d, &dinv);
         bool on = (p.subtype == 1 && d > p.R) || (p.subtype == 2 && d < p.R && d > 0.0f);
         if (on) {
-            float pre = p.k * (d - p.R) / d;
+            float pre = p.k * (d - p.R) * dinv;
             gx += pre * vx; gy += pre * vy; gz += pre * vz;
             if (ENERGY) *e += 0.5f * p.k * (d - p.R) * (d - p.R);
         }

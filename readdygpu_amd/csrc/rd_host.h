@@ -9,6 +9,7 @@
 
 #include "../../include/rdgpu.h"
 #include "rd_device.h"
+#include "rd_tiled.h"
 
 struct RdStepState;
 
@@ -78,6 +79,13 @@ struct rd_system {
     // captured step
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+
+    // force-kernel launch configuration (autotuned)
+    rd_launch_config lc{};        // requested
+    bool tiled = false;           // chosen: LDS-tiled kernel
+    int tile_block = 256;
+    RdTileGeom tg{};
+    size_t tile_lds = 0;
 };
 
 double rd_host_r0(const rd_system* h, int i, int j);

@@ -83,55 +83,128 @@ __device__ __forceinline__ long long to_fix(float v)
     return (long long)(v * RD_FIX_SCALE);
 }
 
+// Per-block LDS copy of the type-pair tables and EM coefficients: the inner
+// loop then reads them with LDS latency instead of a global round trip.
+struct RdTab {
+    float rc2[RD_MAXT * RD_MAXT];
+    float r0[RD_MAXT * RD_MAXT];
+    uint32_t mask[RD_MAXT * RD_MAXT];
+    float a[RD_MAXT], b[RD_MAXT], prad[RD_MAXT];
+    RdPot2Dev pot2[RD_MAXP2];
+};
+
+// all threads of the block must call this (it ends with a barrier)
+__device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
+{
+    const int n2 = P->nt * P->nt;
+    for (int k = threadIdx.x; k < n2; k += blockDim.x) {
+        T.rc2[k] = P->rc2[k];
+        T.r0[k] = P->r0[k];
+        T.mask[k] = P->p2mask[k];
+    }
+    for (int k = threadIdx.x; k < RD_MAXT; k += blockDim.x) {
+        T.a[k] = P->a[k];
+        T.b[k] = P->b[k];
+        T.prad[k] = P->prad[k];
+    }
+    const int nw = (int)(sizeof(RdPot2Dev) / 4) * RD_MAXP2;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x)
+        reinterpret_cast<uint32_t*>(T.pot2)[k] = reinterpret_cast<const uint32_t*>(P->pot2)[k];
+    __syncthreads();
+}
+
+// accumulate the pair (p, q) into the fixed-point gradient of p
+template <bool ENERGY>
+__device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p, float4 q, float Lbx, float Lby,
+                                           float Lbz, float hx, float hy, float hz, long long& ax, long long& ay,
+                                           long long& az, float& e)
+{
+    float dx = min_image(p.x - q.x, Lbx, hx);
+    float dy = min_image(p.y - q.y, Lby, hy);
+    float dz = min_image(p.z - q.z, Lbz, hz);
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    const int tj = w_type(q.w);
+    if (d2 < T.rc2[rowbase + tj] && d2 > 0.0f) {
+        float rinv = __builtin_amdgcn_rsqf(d2);
+        float r = d2 * rinv;
+        float r0 = T.r0[rowbase + tj];
+        uint32_t m = T.mask[rowbase + tj];
+        float pre = 0.0f, ep = 0.0f;
+        while (m) {
+            int pi = __builtin_ctz(m);
+            m &= m - 1;
+            pre += pair_pre_fast<ENERGY>(T.pot2[pi], r0, r, rinv, &ep);
+        }
+        ax += to_fix(pre * dx); ay += to_fix(pre * dy); az += to_fix(pre * dz);
+        if (ENERGY) e += 0.5f * ep;
+    }
+}
+
 // gradient of U (pair + external) for the particle in slot s
 template <bool ENERGY>
-__device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const RdPot1Dev* __restrict__ pot1,
+__device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const RdTab& T,
+                                              const RdPot1Dev* __restrict__ pot1,
                                               const float4* __restrict__ ps, const uint32_t* __restrict__ start,
                                               uint32_t s, float4 p, float* energy)
 {
     const int ti = w_type(p.w);
     const int nt = P->nt;
+    const int rowbase = ti * nt;
     long long ax = 0, ay = 0, az = 0;
     float e = 0.0f;
-    int cx = cell_coord(p.x, P->lo[0], P->cell, P->L[0]);
-    int cy = cell_coord(p.y, P->lo[1], P->cell, P->L[1]);
-    int cz = cell_coord(p.z, P->lo[2], P->cell, P->L[2]);
+    const int Lx = P->L[0], Ly = P->L[1], Lz = P->L[2];
+    int cx = cell_coord(p.x, P->lo[0], P->cell, Lx);
+    int cy = cell_coord(p.y, P->lo[1], P->cell, Ly);
+    int cz = cell_coord(p.z, P->lo[2], P->cell, Lz);
     const float Lbx = P->Lb[0], Lby = P->Lb[1], Lbz = P->Lb[2];
     const float hx = P->half[0], hy = P->half[1], hz = P->half[2];
-    const float* rc2row = &P->rc2[ti * nt];
-    const float* r0row = &P->r0[ti * nt];
-    const uint32_t* mrow = &P->p2mask[ti * nt];
+    auto body = [&](uint32_t j, float4 q) {
+        if (j != s) pair_accum<ENERGY>(T, rowbase, p, q, Lbx, Lby, Lbz, hx, hy, hz, ax, ay, az, e);
+    };
+    auto run = [&](uint32_t j, uint32_t j1) {
+        for (; j + 1 < j1; j += 2) {
+            float4 q0 = ps[j], q1 = ps[j + 1];
+            body(j, q0);
+            body(j + 1, q1);
+        }
+        if (j < j1) body(j, ps[j]);
+    };
     if (P->n_pot2 > 0) {
-        for_stencil(P, start, cx, cy, cz, P->S, [&](uint32_t j) {
-            if (j == s) return;
-            float4 q = ps[j];
-            float dx = min_image(p.x - q.x, Lbx, hx);
-            float dy = min_image(p.y - q.y, Lby, hy);
-            float dz = min_image(p.z - q.z, Lbz, hz);
-            float d2 = dx * dx;
-            d2 = d2 + dy * dy;
-            d2 = d2 + dz * dz;
-            int tj = w_type(q.w);
-            if (d2 < rc2row[tj] && d2 > 0.0f) {
-                float r = sqrtf(d2);
-                float r0 = r0row[tj];
-                uint32_t m = mrow[tj];
-                float pre = 0.0f, ep = 0.0f;
-                while (m) {
-                    int pi = __builtin_ctz(m);
-                    m &= m - 1;
-                    pre += pair_pre<ENERGY>(P->pot2[pi], r0, r, &ep);
+        if (P->S == 1 && Lx >= 3 && Ly >= 3 && Lz >= 3) {
+            // 27-cell stencil as 9 rows of contiguous slots; the row starts of
+            // the next row are loaded while the current row is processed
+            int a0, b0, a1, b1;
+            const int nr = x_runs(cx, 1, Lx, &a0, &b0, &a1, &b1);
+            auto rowof = [&](int r) {
+                int yy = cy + (r % 3) - 1, zz = cz + (r / 3) - 1;
+                yy += yy < 0 ? Ly : 0; yy -= yy >= Ly ? Ly : 0;
+                zz += zz < 0 ? Lz : 0; zz -= zz >= Lz ? Lz : 0;
+                return (zz * Ly + yy) * Lx;
+            };
+            int row = rowof(0);
+            uint32_t nb0 = start[row + a0], ne0 = start[row + b0 + 1], nb1 = 0, ne1 = 0;
+            if (nr == 2) { nb1 = start[row + a1]; ne1 = start[row + b1 + 1]; }
+            for (int r = 0; r < 9; ++r) {
+                uint32_t cb0 = nb0, ce0 = ne0, cb1 = nb1, ce1 = ne1;
+                if (r < 8) {
+                    row = rowof(r + 1);
+                    nb0 = start[row + a0]; ne0 = start[row + b0 + 1];
+                    if (nr == 2) { nb1 = start[row + a1]; ne1 = start[row + b1 + 1]; }
                 }
-                ax += to_fix(pre * dx); ay += to_fix(pre * dy); az += to_fix(pre * dz);
-                if (ENERGY) e += 0.5f * ep;
+                run(cb0, ce0);
+                run(cb1, ce1);
             }
-        });
+        } else {
+            for_stencil(P, start, cx, cy, cz, P->S, [&](uint32_t j) { body(j, ps[j]); });
+        }
     }
     float gx = (float)((double)ax * RD_FIX_INV);
     float gy = (float)((double)ay * RD_FIX_INV);
     float gz = (float)((double)az * RD_FIX_INV);
     const int np1 = P->n_pot1;
-    const float prad = P->prad[ti];
+    const float prad = T.prad[ti];
     for (int k = 0; k < np1; ++k) {
         const RdPot1Dev& q = pot1[k];
         if (!((q.mask >> ti) & 1u)) continue;
@@ -154,4 +227,42 @@ __device__ __forceinline__ uint32_t cell_key(const RdDev* __restrict__ P, float 
     int cy = cell_coord(y, P->lo[1], P->cell, P->L[1]);
     int cz = cell_coord(z, P->lo[2], P->cell, P->L[2]);
     return (uint32_t)((cz * P->L[1] + cy) * P->L[0] + cx);
+}
+
+// Euler-Maruyama for slot s (CudaSimulation.cu:1850-1858): noise, x += -a*dU
+// + b*xi, one periodic correction, the next cell key and its histogram entry.
+__device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, const RdTab& T,
+                                                RdStepState* __restrict__ st, const float* __restrict__ noise,
+                                                int n, uint32_t s, float4 p, float3 g, float4* __restrict__ pn,
+                                                uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                uint32_t* __restrict__ count)
+{
+    const int t = w_type(p.w);
+    const uint32_t idx = w_index(p.w);
+    const unsigned long long step = st->step;
+    float3 z;
+    if (st->noise_nsteps == 0) {
+        z = rd_normal3(P->seed, P->stream, step, idx);
+    } else {
+        long long ls = (long long)(step - st->noise_step0);
+        if (ls >= 0 && ls < st->noise_nsteps) {
+            size_t base = (size_t)ls * 3 * (size_t)n + idx;
+            z = make_float3(noise[base], noise[base + n], noise[base + 2 * (size_t)n]);
+        } else {
+            z = make_float3(0.0f, 0.0f, 0.0f);
+            atomicOr(&st->err, 2u);
+        }
+    }
+    const float a = T.a[t], bb = T.b[t];
+    float x = (p.x - a * g.x) + bb * z.x;
+    float y = (p.y - a * g.y) + bb * z.y;
+    float zc = (p.z - a * g.z) + bb * z.z;
+    x = wrap_coord(x, P->lo[0], P->hi[0], P->Lb[0]);
+    y = wrap_coord(y, P->lo[1], P->hi[1], P->Lb[1]);
+    zc = wrap_coord(zc, P->lo[2], P->hi[2], P->Lb[2]);
+    if (!(fabsf(x) < 3.0e38f && fabsf(y) < 3.0e38f && fabsf(zc) < 3.0e38f)) atomicOr(&st->err, 1u);
+    pn[s] = make_float4(x, y, zc, p.w);
+    uint32_t k = cell_key(P, x, y, zc);
+    key[s] = k;
+    rank[s] = atomicAdd(&count[k], 1u);
 }

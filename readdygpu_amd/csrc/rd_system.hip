@@ -44,39 +44,14 @@ __global__ __launch_bounds__(RD_BLOCK) void k_force_update(const float4* __restr
                                                            RdStepState* __restrict__ st,
                                                            const float* __restrict__ noise, int n)
 {
+    __shared__ RdTab T;
+    load_tab(T, P);
     int b = xcd_block(blockIdx.x, gridDim.x);
     uint32_t s = (uint32_t)b * RD_BLOCK + threadIdx.x;
     if (s >= (uint32_t)n) return;
     float4 p = ps[s];
-    float3 g = gather_grad<false>(P, pot1, ps, start, s, p, nullptr);
-    const int t = w_type(p.w);
-    const uint32_t idx = w_index(p.w);
-    const unsigned long long step = st->step;
-    float3 z;
-    if (st->noise_nsteps == 0) {
-        z = rd_normal3(P->seed, P->stream, step, idx);
-    } else {
-        long long ls = (long long)(step - st->noise_step0);
-        if (ls >= 0 && ls < st->noise_nsteps) {
-            size_t base = (size_t)ls * 3 * (size_t)n + idx;
-            z = make_float3(noise[base], noise[base + n], noise[base + 2 * (size_t)n]);
-        } else {
-            z = make_float3(0.0f, 0.0f, 0.0f);
-            atomicOr(&st->err, 2u);
-        }
-    }
-    const float a = P->a[t], bb = P->b[t];
-    float x = (p.x - a * g.x) + bb * z.x;
-    float y = (p.y - a * g.y) + bb * z.y;
-    float zc = (p.z - a * g.z) + bb * z.z;
-    x = wrap_coord(x, P->lo[0], P->hi[0], P->Lb[0]);
-    y = wrap_coord(y, P->lo[1], P->hi[1], P->Lb[1]);
-    zc = wrap_coord(zc, P->lo[2], P->hi[2], P->Lb[2]);
-    if (!(fabsf(x) < 3.0e38f && fabsf(y) < 3.0e38f && fabsf(zc) < 3.0e38f)) atomicOr(&st->err, 1u);
-    pn[s] = make_float4(x, y, zc, p.w);
-    uint32_t k = cell_key(P, x, y, zc);
-    key[s] = k;
-    rank[s] = atomicAdd(&count[k], 1u);
+    float3 g = gather_grad<false>(P, T, pot1, ps, start, s, p, nullptr);
+    em_update_store(P, T, st, noise, n, s, p, g, pn, key, rank, count);
 }
 
 // ---- exclusive scan of count[ncell] -> start[ncell+1], zeroing count ----
@@ -248,12 +223,14 @@ __global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict
                                                         float* __restrict__ grad, double* __restrict__ etot)
 {
     __shared__ double red[RD_BLOCK / 64];
+    __shared__ RdTab T;
+    load_tab(T, P);
     uint32_t s = blockIdx.x * RD_BLOCK + threadIdx.x;
     double e = 0.0;
     if (s < (uint32_t)n) {
         float4 p = ps[s];
         float ef = 0.0f;
-        float3 g = gather_grad<true>(P, pot1, ps, start, s, p, &ef);
+        float3 g = gather_grad<true>(P, T, pot1, ps, start, s, p, &ef);
         uint32_t i = w_index(p.w);
         grad[3 * i] = g.x; grad[3 * i + 1] = g.y; grad[3 * i + 2] = g.z;
         e = (double)ef;
@@ -681,6 +658,71 @@ static void launch_sort(rd_system* h, int advance)
 
 void rd_host_launch_force(rd_system* h);
 
+// Tile shape for the LDS-tiled kernel: home particles per tile near
+// fill x block, staged capacity with headroom for density fluctuations,
+// LDS <= 64 KB per workgroup.
+static void choose_tiles(rd_system* h)
+{
+    const RdDev& P = h->P;
+    const int Lx = P.L[0], Ly = P.L[1], Lz = P.L[2];
+    h->tiled = false;
+    h->tile_block = h->lc.block > 0 ? h->lc.block : 256;
+    if (h->lc.kernel == RD_KERNEL_GATHER) return;
+    if (P.S != 1 || Lx < 3 || Ly < 3 || Lz < 3 || P.n_pot2 == 0 || h->n == 0) return;
+    const double rc = (double)h->n / ((double)Lx * Ly * Lz);
+    const double fill = h->lc.fill > 0 ? h->lc.fill : 0.6;
+    const double home = std::max(16.0, h->tile_block * fill);
+    RdTileGeom G{};
+    const int tymax = std::max(1, std::min(8, Ly - 2));
+    const double rowp = Lx * rc;
+    if (Lx <= 1024 && rowp <= home) {
+        G.nseg = 1; G.TW = Lx; G.W = Lx;
+        G.TY = h->lc.rows_per_tile > 0 ? h->lc.rows_per_tile : (int)std::lround(home / std::max(rowp, 1e-9));
+    } else {
+        G.TY = h->lc.rows_per_tile > 0 ? h->lc.rows_per_tile : 1;
+        G.TY = std::min(G.TY, tymax);
+        int tw = (int)std::lround(home / std::max(G.TY * rc, 1e-9));
+        G.TW = std::max(1, std::min(tw, Lx - 2));
+        G.nseg = (Lx + G.TW - 1) / G.TW;
+        G.W = G.TW + 2;
+    }
+    G.TY = std::max(1, std::min(G.TY, tymax));
+    for (;;) {
+        double staged = 3.0 * (G.TY + 2) * G.W * rc;
+        G.cap = (int)((1.5 * staged + 64.0 + 63.0) / 64.0) * 64;
+        size_t lds = rd_tile_lds_bytes(G);
+        if (lds <= 64 * 1024) { h->tile_lds = lds; break; }
+        if (G.TY > 1) { G.TY--; continue; }
+        if (G.nseg == 1 || G.TW <= 1) return;       // cannot fit: gather kernel
+        G.TW = std::max(1, G.TW / 2);
+        G.nseg = (Lx + G.TW - 1) / G.TW;
+        G.W = G.TW + 2;
+    }
+    G.nyt = (Ly + G.TY - 1) / G.TY;
+    G.ntiles = Lz * G.nyt * G.nseg;
+    h->tg = G;
+    h->tiled = true;
+}
+
+template <int MODE>
+static void launch_tiled(rd_system* h, float* grad, double* etot)
+{
+    const RdTileGeom& G = h->tg;
+    dim3 grid(G.ntiles), blk(h->tile_block);
+    size_t lds = h->tile_lds;
+#define RD_TL(B)                                                                                         \
+    hipLaunchKernelGGL((k_force_tiled<B, MODE>), grid, blk, lds, h->stream, h->d_ps, h->d_pn, h->d_key, \
+                       h->d_rank, h->d_count, h->d_start, h->d_P, h->d_pot1, h->d_st, h->d_noise, h->n, G, \
+                       grad, etot)
+    switch (h->tile_block) {
+    case 64: RD_TL(64); break;
+    case 128: RD_TL(128); break;
+    case 512: RD_TL(512); break;
+    default: RD_TL(256); break;
+    }
+#undef RD_TL
+}
+
 static void launch_step(rd_system* h)
 {
     rd_host_launch_force(h);
@@ -694,6 +736,7 @@ static int upload(rd_system* h)
     if (rc) return rc;
     rc = ensure_capacity(h);
     if (rc) return rc;
+    choose_tiles(h);
     HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
     if (h->noise_n >= 0 && h->noise_n != h->n) {
         // the noise array was laid out for another particle count: back to the generator
@@ -773,13 +816,56 @@ extern "C" int rd_simulate(rd_system* h, int nsteps)
 
 // pieces of the step for the measurement entry points (rd_bench.hip)
 int rd_host_refresh(rd_system* h) { return refresh(h); }
+
+// reset the global step counter (noise counter), e.g. after autotuning
+int rd_host_set_step(rd_system* h, uint64_t step)
+{
+    RdStepState st;
+    HIPCK(h, hipMemcpyAsync(&st, h->d_st, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    st.step = step;
+    st.err &= ~2u;
+    HIPCK(h, hipMemcpyAsync(h->d_st, &st, sizeof(st), hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    h->step = step;
+    return RD_OK;
+}
 void rd_host_launch_sort(rd_system* h, int advance) { launch_sort(h, advance); }
 void rd_host_launch_force(rd_system* h)
 {
-    if (h->n > 0)
-        hipLaunchKernelGGL(k_force_update, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_pn,
-                           h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P, h->d_pot1, h->d_st, h->d_noise,
-                           h->n);
+    if (h->n <= 0) return;
+    if (h->tiled) {
+        launch_tiled<RD_TILE_UPDATE>(h, nullptr, nullptr);
+        return;
+    }
+    hipLaunchKernelGGL(k_force_update, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_pn,
+                       h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P, h->d_pot1, h->d_st, h->d_noise, h->n);
+}
+
+extern "C" int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg)
+{
+    if (!h || !cfg) return RD_EINVAL;
+    if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_TILED) return set_err(h, RD_EINVAL, "bad kernel");
+    if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256 && cfg->block != 512)
+        return set_err(h, RD_EINVAL, "block must be 64, 128, 256 or 512");
+    if (cfg->rows_per_tile < 0 || cfg->rows_per_tile > 8 || cfg->fill < 0)
+        return set_err(h, RD_EINVAL, "bad tile shape");
+    h->lc = *cfg;
+    h->dirty = true;
+    return RD_OK;
+}
+
+extern "C" int rd_get_launch_config(rd_system* h, rd_launch_config* cfg)
+{
+    if (!h || !cfg) return RD_EINVAL;
+    *cfg = h->lc;
+    cfg->used_kernel = h->tiled ? RD_KERNEL_TILED : RD_KERNEL_GATHER;
+    cfg->used_rows = h->tiled ? h->tg.TY : 0;
+    cfg->used_segments = h->tiled ? h->tg.nseg : 0;
+    cfg->used_cap = h->tiled ? h->tg.cap : 0;
+    cfg->used_lds = h->tiled ? (long)h->tile_lds : 0;
+    if (h->tiled) cfg->block = h->tile_block;
+    return RD_OK;
 }
 
 extern "C" int rd_copy_pos_from_device(rd_system* h, float* xyz, int* type)
@@ -825,8 +911,11 @@ extern "C" int rd_gradients(rd_system* h, float* grad, double* energy)
     if (rc) return rc;
     if (h->n == 0) { if (energy) *energy = 0.0; return RD_OK; }
     HIPCK(h, hipMemsetAsync(h->d_etot, 0, sizeof(double), h->stream));
-    hipLaunchKernelGGL(k_gradients, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
-                       h->d_P, h->d_pot1, h->n, h->d_xyz_io, h->d_etot);
+    if (h->tiled)
+        launch_tiled<RD_TILE_GRAD>(h, h->d_xyz_io, h->d_etot);
+    else
+        hipLaunchKernelGGL(k_gradients, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps,
+                           h->d_start, h->d_P, h->d_pot1, h->n, h->d_xyz_io, h->d_etot);
     HIPCK(h, hipGetLastError());
     std::vector<float> tmp(3 * (size_t)h->n);
     double e = 0.0;

@@ -1,0 +1,87 @@
+// rd_tune.hip -- autotuner of the lattice cell size and the force-kernel
+// launch shape.  Every candidate computes the same trajectory bit for bit
+// (fixed-point pair sums), so the choice only affects speed.
+#include "rd_host.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+int rd_host_refresh(rd_system* h);
+int rd_host_set_step(rd_system* h, uint64_t step);
+
+static int time_steps(rd_system* h, int nsteps, double* ms_per_step)
+{
+    int rc = rd_simulate(h, 2);          // builds the graph, warms caches
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return RD_EHIP;
+    (void)hipEventRecord(e0, h->stream);
+    for (int i = 0; i < nsteps; ++i) (void)hipGraphLaunch(h->graph_exec, h->stream);
+    (void)hipEventRecord(e1, h->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->step += (uint64_t)nsteps;
+    if (e != hipSuccess) return RD_EHIP;
+    *ms_per_step = ms / nsteps;
+    return RD_OK;
+}
+
+extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
+{
+    if (!h || trial_steps < 1) return RD_EINVAL;
+    int rc = rd_host_refresh(h);
+    if (rc) return rc;
+    if (h->n == 0) { if (best_ms) *best_ms = 0.0; return RD_OK; }
+    // snapshot: positions (index order) and the step counter
+    rc = rd_copy_pos_from_device(h, nullptr, nullptr);
+    if (rc) return rc;
+    const std::vector<float> sx = h->hxyz;
+    const std::vector<int> st = h->htype;
+    const uint64_t s0 = h->step;
+    const double user_cell = h->prm.cell_size;
+    const rd_launch_config user_lc = h->lc;
+
+    std::vector<double> cells;
+    const double base = std::max(h->max_range, h->max_rxn);
+    if (user_cell > 0 || base <= 0) cells.push_back(user_cell > 0 ? user_cell : h->cell);
+    else
+        for (double f : { 1.01, 1.2, 1.45, 1.75, 2.1 }) cells.push_back(f * base);
+    struct Cfg { int kernel, block; double fill; };
+    const Cfg cfgs[] = { { RD_KERNEL_GATHER, 256, 0.0 }, { RD_KERNEL_TILED, 128, 0.6 }, { RD_KERNEL_TILED, 256, 0.6 },
+                         { RD_KERNEL_TILED, 256, 1.0 }, { RD_KERNEL_TILED, 512, 0.8 } };
+    double best = 1e30, best_cell = cells[0];
+    Cfg best_cfg = cfgs[0];
+    for (double c : cells) {
+        for (const Cfg& k : cfgs) {
+            h->prm.cell_size = c;
+            rd_launch_config lc{};
+            lc.kernel = k.kernel; lc.block = k.block; lc.fill = k.fill;
+            rc = rd_set_launch_config(h, &lc);
+            if (rc == RD_OK) rc = rd_set_particles(h, (int)st.size(), sx.data(), st.data());
+            if (rc == RD_OK) rc = rd_host_refresh(h);
+            if (rc == RD_ERANGE) continue;          // too many cells for this size
+            if (rc) return rc;
+            if (k.kernel == RD_KERNEL_TILED && !h->tiled) continue;
+            double ms = 0.0;
+            rc = time_steps(h, trial_steps, &ms);
+            if (rc) return rc;
+            if (ms < best) { best = ms; best_cell = c; best_cfg = k; }
+        }
+    }
+    // keep the winner, restore the state
+    h->prm.cell_size = user_cell > 0 ? user_cell : best_cell;
+    rd_launch_config lc = user_lc;
+    lc.kernel = best_cfg.kernel; lc.block = best_cfg.block; lc.fill = best_cfg.fill;
+    rc = rd_set_launch_config(h, &lc);
+    if (rc == RD_OK) rc = rd_set_particles(h, (int)st.size(), sx.data(), st.data());
+    if (rc == RD_OK) rc = rd_host_refresh(h);
+    if (rc) return rc;
+    rc = rd_host_set_step(h, s0);
+    if (rc) return rc;
+    if (best_ms) *best_ms = best;
+    return RD_OK;
+}

@@ -199,6 +199,23 @@ class Simulation:
         cr = np.ascontiguousarray(creactions, dtype=np.float32)
         self._check(self.lib.rd_apply_reactions(self.h, _ptr(cr)))
 
+    def set_launch_config(self, kernel: int = 0, block: int = 0, rows_per_tile: int = 0, fill: float = 0.0):
+        """Force-kernel launch shape (0 = automatic); the trajectory does not depend on it."""
+        c = _lib.rd_launch_config()
+        c.kernel, c.block, c.rows_per_tile, c.fill = kernel, block, rows_per_tile, fill
+        self._check(self.lib.rd_set_launch_config(self.h, C.byref(c)))
+
+    def launch_config(self) -> dict:
+        c = _lib.rd_launch_config()
+        self._check(self.lib.rd_get_launch_config(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in c._fields_}
+
+    def autotune(self, trial_steps: int = 20) -> float:
+        """Pick the cell size and kernel shape by timing; state is restored."""
+        ms = C.c_double()
+        self._check(self.lib.rd_autotune(self.h, int(trial_steps), C.byref(ms)))
+        return ms.value
+
     def info(self) -> _lib.rd_info:
         i = _lib.rd_info()
         self._check(self.lib.rd_get_info(self.h, C.byref(i)))
