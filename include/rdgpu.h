@@ -161,6 +161,42 @@ int rd_apply_reactions(rd_system* h, const float* creactions);
 
 int rd_get_info(rd_system* h, rd_info* info);
 
+/* ---- input files: the reference's XML schema (Xerces-free reader) ----
+ * rd_model is the parsed Simulation of Simulation::readXML / readGlobalXML /
+ * parse (Simulation.cpp:206-338): param_global (GlobalParametersHandler.cpp:
+ * 73-143), param_particles (ParticleParametersHandler.cpp:48-162),
+ * tplgy_potentials (PotentialParametersHandler.cpp:38-83), tplgy_coordinates
+ * (CoordReader.cpp:16-65).  Parsing needs no GPU. */
+typedef struct rd_model rd_model;
+typedef struct {
+    int n_particles;
+    int n_types;
+    int n_pot1, n_pot2;
+    int n_groups;            /* group instances found (bonded groups are not simulated) */
+    int frames;              /* nSimulationSteps */
+    int steps_per_frame;     /* round(dt / dtOpenMM) (XMLReader.cpp:67) */
+    double dt_frame;         /* dt */
+    double dt_step;          /* dtOpenMM / dt_dynamic / dt_dyn */
+    double temperature, boltzmann;
+    double lattice_bounds[6];
+    int device;              /* cudaDeviceIndex / cudaDevice */
+    int testmode;
+} rd_model_info;
+int rd_model_read_xml(const char* param_particles, const char* tplgy_potentials, const char* param_groups,
+                      const char* tplgy_coordinates, const char* tplgy_groups, const char* param_global,
+                      rd_model** out, char* err, int err_len);
+void rd_model_free(rd_model* m);
+int rd_model_get_info(const rd_model* m, rd_model_info* info);
+/* D, default radius [nt]; radii matrix [nt*nt] (0 = default) */
+int rd_model_types(const rd_model* m, double* D, double* radius, double* radii_matrix);
+/* positions [3n], types [n], ReaDDy particle ids [n] in file order */
+int rd_model_particles(const rd_model* m, float* xyz, int* type, int* readdy_id);
+int rd_model_pot1(const rd_model* m, int i, rd_pot1* pot);
+int rd_model_pot2(const rd_model* m, int i, rd_pot2* pot, uint8_t* pair_mask);
+/* Simulation + CudaSimulation::initialize from a parsed model (maxCutoff as
+ * given; the reference JNI path hard-codes 10, lib/ReaDDyGPU.cpp:101) */
+int rd_create_from_model(const rd_model* m, double max_cutoff, uint64_t seed, rd_system** out);
+
 /* ---- launch configuration of the force/update kernel.  No counterpart in
  *      the reference (its launch shape is fixed: 128 threads per particle
  *      block, 32 per cell, CudaSimulation.cu:193, 2347).  Any configuration

@@ -66,6 +66,25 @@ class rd_launch_config(C.Structure):
     ]
 
 
+class rd_model_info(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_int),
+        ("n_types", C.c_int),
+        ("n_pot1", C.c_int),
+        ("n_pot2", C.c_int),
+        ("n_groups", C.c_int),
+        ("frames", C.c_int),
+        ("steps_per_frame", C.c_int),
+        ("dt_frame", C.c_double),
+        ("dt_step", C.c_double),
+        ("temperature", C.c_double),
+        ("boltzmann", C.c_double),
+        ("lattice_bounds", C.c_double * 6),
+        ("device", C.c_int),
+        ("testmode", C.c_int),
+    ]
+
+
 class rd_info(C.Structure):
     _fields_ = [
         ("n", C.c_int),
@@ -107,6 +126,14 @@ _SIGS = {
     "rd_set_launch_config": (C.c_int, [P, C.POINTER(rd_launch_config)]),
     "rd_get_launch_config": (C.c_int, [P, C.POINTER(rd_launch_config)]),
     "rd_autotune": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
+    "rd_model_read_xml": (C.c_int, [C.c_char_p] * 6 + [C.POINTER(P), C.c_char_p, C.c_int]),
+    "rd_model_free": (None, [P]),
+    "rd_model_get_info": (C.c_int, [P, C.POINTER(rd_model_info)]),
+    "rd_model_types": (C.c_int, [P, P, P, P]),
+    "rd_model_particles": (C.c_int, [P, P, P, P]),
+    "rd_model_pot1": (C.c_int, [P, C.c_int, C.POINTER(rd_pot1)]),
+    "rd_model_pot2": (C.c_int, [P, C.c_int, C.POINTER(rd_pot2), P]),
+    "rd_create_from_model": (C.c_int, [P, C.c_double, C.c_uint64, C.POINTER(P)]),
     "rd_bench_steps": (C.c_int, [P, C.c_int, P]),
     "rd_bench_copy": (C.c_int, [P, C.c_long, C.c_int, C.POINTER(C.c_double)]),
 }

@@ -1,0 +1,189 @@
+// rd_jni.cpp -- JNI entry point of libCReaDDyGPU.so (lib/ReaDDyGPU.cpp:31-503).
+//
+// Same protocol as the reference: read the six input files, then for every
+// frame run stepsPerFrame BD steps, hand the positions to
+// TopGPU.frameCallback(int frame, float[] JPos) with
+// JPos = [N, (x, y, z, index, readdyId) x N] (lib/ReaDDyGPU.cpp:264-292), and
+// apply the reactions Java leaves in the field float[] cReactions =
+// [count, (readdyId, newType, x, y, z, index) x count] (lib/ReaDDyGPU.cpp:
+// 303-463).  Errors are printed and end the call, as in the reference.
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "../../include/rdgpu.h"
+#include "../../include/readdy_jni.h"
+
+namespace {
+
+// messages go to stdout, flushed at once as the reference's cout << endl
+void say(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void say(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vprintf(fmt, ap);
+    va_end(ap);
+    fflush(stdout);
+}
+
+typedef jclass (*GetObjectClass_t)(JNIEnv*, jobject);
+typedef jmethodID (*GetMethodID_t)(JNIEnv*, jclass, const char*, const char*);
+typedef jboolean (*CallBooleanMethod_t)(JNIEnv*, jobject, jmethodID, ...);
+typedef jfieldID (*GetFieldID_t)(JNIEnv*, jclass, const char*, const char*);
+typedef jobject (*GetObjectField_t)(JNIEnv*, jobject, jfieldID);
+typedef const char* (*GetStringUTFChars_t)(JNIEnv*, jstring, jboolean*);
+typedef void (*ReleaseStringUTFChars_t)(JNIEnv*, jstring, const char*);
+typedef jsize (*GetArrayLength_t)(JNIEnv*, jarray);
+typedef jfloatArray (*NewFloatArray_t)(JNIEnv*, jsize);
+typedef void* (*GetPrimitiveArrayCritical_t)(JNIEnv*, jarray, jboolean*);
+typedef void (*ReleasePrimitiveArrayCritical_t)(JNIEnv*, jarray, void*, jint);
+typedef void (*DeleteLocalRef_t)(JNIEnv*, jobject);
+
+template <class T>
+T fn(JNIEnv* env, int idx)
+{
+    return reinterpret_cast<T>(const_cast<void*>((*env)[idx]));
+}
+
+bool jstr(JNIEnv* env, jstring s, std::string& out)
+{
+    if (!s) return false;
+    const char* c = fn<GetStringUTFChars_t>(env, RD_JNI_GetStringUTFChars)(env, s, nullptr);
+    if (!c) return false;
+    out = c;
+    fn<ReleaseStringUTFChars_t>(env, RD_JNI_ReleaseStringUTFChars)(env, s, c);
+    return true;
+}
+
+}  // namespace
+
+extern "C" void Java_readdy_impl_sim_top_TopGPU_cCreateSimulation(JNIEnv* env, jobject obj, jboolean testmode,
+                                                                 jstring paramParticles, jstring tplgyPotentials,
+                                                                 jstring paramGroups, jstring tplgyCoord,
+                                                                 jstring tplgyGroups, jstring paramGlobal)
+{
+    (void)testmode;
+    std::string p_part, p_pot, p_grp, p_crd, p_tgrp, p_glob;
+    if (!jstr(env, paramParticles, p_part) || !jstr(env, tplgyPotentials, p_pot) || !jstr(env, paramGroups, p_grp) ||
+        !jstr(env, tplgyCoord, p_crd) || !jstr(env, tplgyGroups, p_tgrp) || !jstr(env, paramGlobal, p_glob)) {
+        say("error while reading the input file paths\n");
+        return;
+    }
+    rd_model* model = nullptr;
+    char err[512] = { 0 };
+    if (rd_model_read_xml(p_part.c_str(), p_pot.c_str(), p_grp.c_str(), p_crd.c_str(), p_tgrp.c_str(),
+                          p_glob.c_str(), &model, err, (int)sizeof(err)) != RD_OK) {
+        say("error while reading the input files: %s\n", err);
+        return;
+    }
+    rd_model_info info;
+    rd_model_get_info(model, &info);
+    if (info.n_groups > 0)
+        say("warning: %d particle groups found; bonded group forces are not simulated\n", info.n_groups);
+
+    // lib/ReaDDyGPU.cpp:101: maxCutoff = 10; the reference seeds its RNG with
+    // the wall clock (CudaSimulation.cu:773), RDGPU_SEED fixes the seed.
+    const char* seed_env = getenv("RDGPU_SEED");
+    uint64_t seed = seed_env ? strtoull(seed_env, nullptr, 10) : (uint64_t)time(nullptr);
+    rd_system* h = nullptr;
+    if (rd_create_from_model(model, 10.0, seed, &h) != RD_OK) {
+        say("error during initialization\n");
+        rd_model_free(model);
+        return;
+    }
+    std::vector<int> rid(info.n_particles);
+    rd_model_particles(model, nullptr, nullptr, rid.data());
+    rd_model_free(model);
+
+    jclass cls = fn<GetObjectClass_t>(env, RD_JNI_GetObjectClass)(env, obj);
+    jmethodID mid = cls ? fn<GetMethodID_t>(env, RD_JNI_GetMethodID)(env, cls, "frameCallback", "(I[F)Z") : nullptr;
+    if (!mid) {
+        say("mthod not found\n");
+        rd_destroy(h);
+        return;
+    }
+    jfieldID fid = fn<GetFieldID_t>(env, RD_JNI_GetFieldID)(env, cls, "cReactions", "[F");
+    say("%d frames with each %d steps\n", info.frames, info.steps_per_frame);
+
+    std::vector<float> xyz;
+    std::vector<int> type;
+    for (int frame = 0; frame < info.frames; ++frame) {
+        if (rd_simulate(h, info.steps_per_frame) != RD_OK) {
+            say("error: %s\n", rd_last_error(h));
+            break;
+        }
+        say("frame: %d (step %lld)\n", frame, (long long)frame * info.steps_per_frame);
+        rd_info si;
+        rd_get_info(h, &si);
+        const int n = si.n;
+        xyz.resize(3 * (size_t)n);
+        type.resize(n);
+        if (rd_copy_pos_from_device(h, xyz.data(), type.data()) != RD_OK) {
+            say("error while copying positions from device\n");
+            break;
+        }
+        // JPos = [N, (x, y, z, index, readdyId) x N]
+        jfloatArray jpos = fn<NewFloatArray_t>(env, RD_JNI_NewFloatArray)(env, (jsize)(5 * n + 1));
+        float* pos = jpos ? (float*)fn<GetPrimitiveArrayCritical_t>(env, RD_JNI_GetPrimitiveArrayCritical)(env, jpos, nullptr)
+                          : nullptr;
+        if (!pos) {
+            say("cant get array pointer\n");
+            break;
+        }
+        pos[0] = (float)n;
+        for (int j = 0; j < n; ++j) {
+            pos[1 + 5 * j] = xyz[3 * j];
+            pos[2 + 5 * j] = xyz[3 * j + 1];
+            pos[3 + 5 * j] = xyz[3 * j + 2];
+            pos[4 + 5 * j] = (float)j;
+            pos[5 + 5 * j] = (float)rid[j];
+        }
+        fn<ReleasePrimitiveArrayCritical_t>(env, RD_JNI_ReleasePrimitiveArrayCritical)(env, jpos, pos, 0);
+        jboolean ok = fn<CallBooleanMethod_t>(env, RD_JNI_CallBooleanMethod)(env, obj, mid, (jint)frame, jpos);
+        fn<DeleteLocalRef_t>(env, RD_JNI_DeleteLocalRef)(env, jpos);
+        if (!ok) {
+            say("error in java\n");
+            break;
+        }
+        // reactions from the Java side
+        if (!fid) continue;
+        jobject arr = fn<GetObjectField_t>(env, RD_JNI_GetObjectField)(env, obj, fid);
+        if (!arr) continue;
+        jsize len = fn<GetArrayLength_t>(env, RD_JNI_GetArrayLength)(env, arr);
+        std::vector<float> cr;
+        float* r = (float*)fn<GetPrimitiveArrayCritical_t>(env, RD_JNI_GetPrimitiveArrayCritical)(env, arr, nullptr);
+        if (r && len > 0) {
+            int cnt = (int)r[0];
+            if (cnt > 0 && 1 + 6 * (long)cnt <= len) cr.assign(r, r + 1 + 6 * (size_t)cnt);
+            else if (cnt > 0) say("error: cReactions shorter than its count\n");
+        }
+        if (r) fn<ReleasePrimitiveArrayCritical_t>(env, RD_JNI_ReleasePrimitiveArrayCritical)(env, arr, r, 0);
+        fn<DeleteLocalRef_t>(env, RD_JNI_DeleteLocalRef)(env, arr);
+        if (cr.empty()) continue;
+        // same compaction as rd_apply_reactions, applied to the ReaDDy ids
+        std::vector<int> keep(rid.begin(), rid.end());
+        std::vector<char> dead(n, 0);
+        const int cnt = (int)cr[0];
+        std::vector<int> born;
+        for (int k = 0; k < cnt; ++k) {
+            const float* e = &cr[1 + 6 * k];
+            int index = (int)e[5], newType = (int)e[1];
+            if (index >= 0 && index < n) dead[index] = 1;
+            if (newType != -1) born.push_back((int)e[0]);
+        }
+        if (rd_apply_reactions(h, cr.data()) != RD_OK) {
+            say("error: %s\n", rd_last_error(h));
+            break;
+        }
+        rid.clear();
+        for (int j = 0; j < n; ++j)
+            if (!dead[j]) rid.push_back(keep[j]);
+        rid.insert(rid.end(), born.begin(), born.end());
+    }
+    rd_destroy(h);
+}
