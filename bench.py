@@ -152,7 +152,7 @@ def main():
             "n_particles": n,
             "cell_size_nm": info.cell_size,
             "lattice": list(info.lattice),
-            "force_kernel": "tiled" if lc["used_kernel"] == 2 else "gather",
+            "force_kernel": {1: "gather", 2: "tiled", 3: "wave"}.get(lc["used_kernel"], "?"),
             "block": lc["block"],
             "rows_per_tile": lc["used_rows"],
         },

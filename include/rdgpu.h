@@ -89,6 +89,8 @@ typedef struct {
     int stencil;           /* pair stencil radius in cells */
     uint64_t step;         /* global step counter */
     long rdf_frames;
+    int n_replicas;        /* independent systems held by the handle (1: plain) */
+    int n_cells;           /* cells of all replicas */
 } rd_info;
 
 /* ---- set-up: Simulation::parse (Simulation.cpp:206-280) and
@@ -113,6 +115,18 @@ int rd_set_reaction_radii(rd_system* h, const double* radius_matrix);
 /* CoordReader (CoordReader.cpp:16-65) + copyPosToDevice/copyTypesToDevice
  * (CudaSimulation.cu:905-970).  Particle index = position in these arrays. */
 int rd_set_particles(rd_system* h, int n, const float* xyz, const int* type);
+/* Batched replicas (no counterpart in the reference, which runs one system per
+ * process: the Simulation object of Simulation.hpp:138-181 holds one box).
+ * n_replicas independent periodic boxes share the particle types and
+ * potentials of the handle; replica r has box bounds[6r..6r+5] (lo,hi per
+ * axis), noise stream streams[r] (NULL: r) and counts[r] particles, stored
+ * consecutively in xyz/type.  Global particle index = offset of the replica
+ * + local index.  One step advances every replica; the noise of a particle
+ * depends only on (seed, stream, step, local index), so a replica evolves
+ * bit for bit as the same system run alone with params.stream = its stream.
+ * Reactions need a single-system handle. */
+int rd_set_replicas(rd_system* h, int n_replicas, const double* bounds, const uint32_t* streams,
+                    const int* counts, const float* xyz, const int* type);
 /* CudaSimulation::initialize: upload tables, build the first cell list */
 int rd_initialize(rd_system* h);
 
@@ -141,7 +155,8 @@ int rd_set_noise_array(rd_system* h, const float* noise, int nsteps);
  *      normalizeRDF (Simulation.cpp:40-62) ---- */
 int rd_rdf_configure(rd_system* h, double r_max, int n_bins);
 int rd_rdf_calculate(rd_system* h);
-/* counts[(ti*nt+tj)*n_bins+bin] over frames so far (ordered pairs) */
+/* counts[((r*nt+ti)*nt+tj)*n_bins+bin] over frames so far (ordered pairs),
+ * r = replica (0 only for a plain system) */
 int rd_rdf_copy(rd_system* h, uint64_t* counts, long* frames);
 /* g(r) in FP64 with the shell volume and the box volume */
 int rd_rdf_normalize(rd_system* h, double* g);
@@ -202,9 +217,10 @@ int rd_create_from_model(const rd_model* m, double max_cutoff, uint64_t seed, rd
  *      block, 32 per cell, CudaSimulation.cu:193, 2347).  Any configuration
  *      gives bitwise the same trajectory: pair gradients are summed in fixed
  *      point, so only speed changes. ---- */
-enum { RD_KERNEL_AUTO = 0, RD_KERNEL_GATHER = 1, RD_KERNEL_TILED = 2 };
+enum { RD_KERNEL_AUTO = 0, RD_KERNEL_GATHER = 1, RD_KERNEL_TILED = 2, RD_KERNEL_WAVE = 3 };
 typedef struct {
-    int kernel;            /* RD_KERNEL_*; AUTO picks TILED when the lattice allows */
+    int kernel;            /* RD_KERNEL_*: GATHER per-particle global gather; TILED cell-row tiles
+                              in LDS; WAVE per-wave shifted slot ranges in LDS.  AUTO = WAVE */
     int block;             /* threads per workgroup of the tiled kernel: 64..1024 */
     int rows_per_tile;     /* cell rows per tile; 0 = from the density */
     double fill;           /* target home particles per tile / block (0 = 0.6) */

@@ -94,6 +94,8 @@ class rd_info(C.Structure):
         ("stencil", C.c_int),
         ("step", C.c_uint64),
         ("rdf_frames", C.c_long),
+        ("n_replicas", C.c_int),
+        ("n_cells", C.c_int),
     ]
 
 
@@ -107,6 +109,7 @@ _SIGS = {
     "rd_add_order_two_potential": (C.c_int, [P, C.POINTER(rd_pot2), P]),
     "rd_set_reaction_radii": (C.c_int, [P, P]),
     "rd_set_particles": (C.c_int, [P, C.c_int, P, P]),
+    "rd_set_replicas": (C.c_int, [P, C.c_int, P, P, P, P, P]),
     "rd_initialize": (C.c_int, [P]),
     "rd_simulate": (C.c_int, [P, C.c_int]),
     "rd_copy_pos_from_device": (C.c_int, [P, P, P]),

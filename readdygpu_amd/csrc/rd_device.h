@@ -31,15 +31,29 @@ struct RdPot1Dev {
     float pad;
 };
 
-// per-system constants, read through one pointer (scalar loads)
-struct RdDev {
+// Geometry of one replica (an independent periodic box).  A plain system is
+// one replica.  Replicas occupy consecutive slot ranges [off, off+n) and
+// consecutive cell ranges [coff, coff + Lx*Ly*Lz) of the shared arrays, so
+// one counting sort over all cells keeps every replica in its own range.
+struct RdRep {
     float lo[3], hi[3], Lb[3], half[3];
     float cell;
     int L[3];
+    int coff;           // first global cell
+    int off;            // first global slot / particle index
+    int n;              // particles
+    uint32_t stream;    // noise stream (replica id)
+};
+
+// one wave of work: 64 consecutive slots of one replica
+struct RdWave {
+    int rep, s0, s1, pad;
+};
+
+// per-system constants, read through one pointer (scalar loads)
+struct RdDev {
     int S;              // pair stencil radius in cells
     int nt, n_pot2, n_pot1;
-    uint32_t stream;
-    uint32_t pad0;
     uint64_t seed;
     float a[RD_MAXT];   // dt*D/(kB T)
     float b[RD_MAXT];   // sqrt(2 D dt)
@@ -69,7 +83,7 @@ __device__ __forceinline__ int cell_coord(float x, float lo, float cell, int L)
     if (!(t >= 0.0f)) return 0;
     if (t > 1.0e9f) t = 1.0e9f;
     int i = (int)floorf(t);
-    return i % L;
+    return i < L ? i : i % L;
 }
 
 __device__ __forceinline__ float min_image(float d, float Lb, float h)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -29,8 +30,20 @@ struct rd_system {
     std::vector<RdPot1Dev> pot1dev;
     RdDev P{};
     double cell = 0.0, max_range = 0.0, max_rxn = 0.0;
-    int ncell = 0;
+    int ncell = 0;                // cells of all replicas
     int nscan = 0;
+
+    // replicas: independent systems of one handle, particles concatenated
+    struct Rep {
+        double lo[3], hi[3];
+        uint32_t stream;
+        int n;
+    };
+    std::vector<Rep> reps;
+    std::vector<RdRep> rep_dev;
+    std::vector<RdWave> wave_tab;
+    std::vector<int> hrep_of;
+    int nwaves = 0;
 
     // particles in index order (host mirror)
     int n = 0;
@@ -57,8 +70,13 @@ struct rd_system {
     float* d_xyz_io = nullptr;
     int* d_type_io = nullptr;
     double* d_etot = nullptr;
+    RdRep* d_reps = nullptr;
+    RdWave* d_waves = nullptr;
+    int* d_rep_of = nullptr;
     int cap = 0;
     int ncell_alloc = 0;
+    int nrep_alloc = 0;
+    int nwave_alloc = 0;
     size_t noise_cap = 0;
     int noise_n = -1;             // particle count the noise array was laid out for
     size_t io_cap = 0;
@@ -68,24 +86,34 @@ struct rd_system {
     double rdf_rmax = 0.0;
     int rdf_bins = 0;
     long rdf_frames = 0;
-    double rdf_vol = 1.0;
-    std::vector<double> rdf_npt_sum, rdf_pairs_sum;
+    int rdf_nrep = 0;
+    std::vector<double> rdf_vol, rdf_pairs_sum;
 
     // pair emission
     uint2* d_pairs = nullptr;
     unsigned long long* d_pcount = nullptr;
     size_t pairs_cap = 0;
 
-    // captured step
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    // captured steps: one graph per step count
+    struct Graph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<int, Graph> graphs;
 
     // force-kernel launch configuration (autotuned)
     rd_launch_config lc{};        // requested
     bool tiled = false;           // chosen: LDS-tiled kernel
+    int kern = 3;                 // kernel in use (RD_KERNEL_*)
     int tile_block = 256;
     RdTileGeom tg{};
     size_t tile_lds = 0;
 };
 
 double rd_host_r0(const rd_system* h, int i, int j);
+int rd_host_refresh(rd_system* h);
+int rd_host_set_step(rd_system* h, uint64_t step);
+int rd_host_set_state(rd_system* h, const std::vector<float>& xyz, const std::vector<int>& type);
+int rd_host_step_graph(rd_system* h, int k, hipGraphExec_t* exec);
+void rd_host_launch_force(rd_system* h);
+void rd_host_launch_sort(rd_system* h, int advance);

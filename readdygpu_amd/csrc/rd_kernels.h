@@ -1,14 +1,15 @@
-// rd_kernels.h -- kernels of the plain BD step on gfx950.
+// rd_kernels.h -- building blocks of the BD step kernels on gfx950.
 //
-// Data layout in HBM (one system):
-//   ps[N]   float4, sorted by cell: x, y, z, w = (index << 6) | type
+// Data layout in HBM (all replicas of a handle; a plain system is one):
+//   ps[N]   float4, sorted by cell: x, y, z, w = (local index << 6) | type
 //   pn[N]   float4, positions after the update, in the slot order of ps
-//   key[N]  new cell id of slot s;  rank[N]: rank of slot s inside its cell
-//   count[ncell], start[ncell+1]: counting-sort histogram and offsets
-// One step = force_update (gather forces, Euler-Maruyama, wrap, new cell key,
-// histogram) -> scan (3 phases) -> scatter.  Replaces orderOne, warpOrderTwo
-// and update (CudaSimulation.cu:2309-2476) and the linked-list cell
-// maintenance with spin locks (CudaSimulation.cu:1860-1914).
+//   key[N]  next global cell of slot s;  rank[N]: its rank inside the cell
+//   count[ncell], start[ncell+1]: counting-sort histogram and offsets over the
+//   concatenated cell ranges of all replicas
+// One step = force/update (gather forces, Euler-Maruyama, wrap, next cell key,
+// histogram) -> scan -> scatter.  Replaces orderOne, warpOrderTwo and update
+// (CudaSimulation.cu:2309-2476) and the linked-list cell maintenance with
+// spin locks (CudaSimulation.cu:1860-1914).
 #pragma once
 #include "rd_device.h"
 
@@ -20,7 +21,7 @@ struct RdStepState {
     unsigned long long step;        // global step counter (noise counter)
     unsigned long long noise_step0; // first step held by the noise array
     int noise_nsteps;               // 0: generator
-    unsigned int err;               // bit 0: non-finite position
+    unsigned int err;               // bit 0: non-finite position, bit 1: noise array exhausted
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -41,13 +42,29 @@ __device__ __forceinline__ int x_runs(int cx, int S, int L, int* a0, int* b0, in
     *a0 = lo; *b0 = L - 1; *a1 = 0; *b1 = hi - L; return 2;
 }
 
+__device__ __forceinline__ void cell_coords(const RdRep& R, float x, float y, float z, int& cx, int& cy, int& cz)
+{
+    cx = cell_coord(x, R.lo[0], R.cell, R.L[0]);
+    cy = cell_coord(y, R.lo[1], R.cell, R.L[1]);
+    cz = cell_coord(z, R.lo[2], R.cell, R.L[2]);
+}
+
+// global cell of a position: the replica's first cell + the reference field
+// index (CudaSimulation.cu:586-588)
+__device__ __forceinline__ uint32_t cell_key(const RdRep& R, float x, float y, float z)
+{
+    int cx, cy, cz;
+    cell_coords(R, x, y, z, cx, cy, cz);
+    return (uint32_t)(R.coff + (cz * R.L[1] + cy) * R.L[0] + cx);
+}
+
 // Visit every slot j of the cells within S of (cx,cy,cz).  Rows along x are
 // contiguous in the sorted arrays, so each row is one or two slot ranges.
 template <class F>
-__device__ __forceinline__ void for_stencil(const RdDev* __restrict__ P, const uint32_t* __restrict__ start,
-                                            int cx, int cy, int cz, int S, F&& f)
+__device__ __forceinline__ void for_stencil(const RdRep& R, const uint32_t* __restrict__ start, int cx, int cy,
+                                            int cz, int S, F&& f)
 {
-    const int Lx = P->L[0], Ly = P->L[1], Lz = P->L[2];
+    const int Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
     int ylo, yhi, zlo, zhi;
     bool fy = 2 * S + 1 >= Ly, fz = 2 * S + 1 >= Lz;
     if (fy) { ylo = 0; yhi = Ly - 1; } else { ylo = -S; yhi = S; }
@@ -60,7 +77,7 @@ __device__ __forceinline__ void for_stencil(const RdDev* __restrict__ P, const u
         for (int oy = ylo; oy <= yhi; ++oy) {
             int yy = fy ? oy : cy + oy;
             if (yy < 0) yy += Ly; else if (yy >= Ly) yy -= Ly;
-            int row = (zz * Ly + yy) * Lx;
+            int row = R.coff + (zz * Ly + yy) * Lx;
             uint32_t j0 = start[row + a0], j1 = start[row + b0 + 1];
             for (uint32_t j = j0; j < j1; ++j) f(j);
             if (nr == 2) {
@@ -115,13 +132,12 @@ __device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
 
 // accumulate the pair (p, q) into the fixed-point gradient of p
 template <bool ENERGY>
-__device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p, float4 q, float Lbx, float Lby,
-                                           float Lbz, float hx, float hy, float hz, long long& ax, long long& ay,
-                                           long long& az, float& e)
+__device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p, float4 q, const RdRep& R,
+                                           long long& ax, long long& ay, long long& az, float& e)
 {
-    float dx = min_image(p.x - q.x, Lbx, hx);
-    float dy = min_image(p.y - q.y, Lby, hy);
-    float dz = min_image(p.z - q.z, Lbz, hz);
+    float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
+    float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
+    float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
     float d2 = dx * dx;
     d2 = d2 + dy * dy;
     d2 = d2 + dz * dz;
@@ -142,26 +158,42 @@ __device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p
     }
 }
 
-// gradient of U (pair + external) for the particle in slot s
+// external potentials, added to the fixed-point pair sum
 template <bool ENERGY>
-__device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const RdTab& T,
+__device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const RdTab& T,
+                                              const RdPot1Dev* __restrict__ pot1, float4 p, long long ax,
+                                              long long ay, long long az, float* e)
+{
+    float gx = (float)((double)ax * RD_FIX_INV);
+    float gy = (float)((double)ay * RD_FIX_INV);
+    float gz = (float)((double)az * RD_FIX_INV);
+    const int ti = w_type(p.w);
+    const int np1 = P->n_pot1;
+    const float prad = T.prad[ti];
+    for (int k = 0; k < np1; ++k) {
+        const RdPot1Dev& q = pot1[k];
+        if (!((q.mask >> ti) & 1u)) continue;
+        ext_grad<ENERGY>(q, prad, p.x, p.y, p.z, gx, gy, gz, e);
+    }
+    return make_float3(gx, gy, gz);
+}
+
+// gradient of U (pair + external) for the particle in slot s, from global
+// memory (the per-particle gather)
+template <bool ENERGY>
+__device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
                                               const RdPot1Dev* __restrict__ pot1,
                                               const float4* __restrict__ ps, const uint32_t* __restrict__ start,
                                               uint32_t s, float4 p, float* energy)
 {
-    const int ti = w_type(p.w);
-    const int nt = P->nt;
-    const int rowbase = ti * nt;
+    const int rowbase = w_type(p.w) * P->nt;
     long long ax = 0, ay = 0, az = 0;
     float e = 0.0f;
-    const int Lx = P->L[0], Ly = P->L[1], Lz = P->L[2];
-    int cx = cell_coord(p.x, P->lo[0], P->cell, Lx);
-    int cy = cell_coord(p.y, P->lo[1], P->cell, Ly);
-    int cz = cell_coord(p.z, P->lo[2], P->cell, Lz);
-    const float Lbx = P->Lb[0], Lby = P->Lb[1], Lbz = P->Lb[2];
-    const float hx = P->half[0], hy = P->half[1], hz = P->half[2];
+    const int Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
+    int cx, cy, cz;
+    cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
     auto body = [&](uint32_t j, float4 q) {
-        if (j != s) pair_accum<ENERGY>(T, rowbase, p, q, Lbx, Lby, Lbz, hx, hy, hz, ax, ay, az, e);
+        if (j != s) pair_accum<ENERGY>(T, rowbase, p, q, R, ax, ay, az, e);
     };
     auto run = [&](uint32_t j, uint32_t j1) {
         for (; j + 1 < j1; j += 2) {
@@ -181,7 +213,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                 int yy = cy + (r % 3) - 1, zz = cz + (r / 3) - 1;
                 yy += yy < 0 ? Ly : 0; yy -= yy >= Ly ? Ly : 0;
                 zz += zz < 0 ? Lz : 0; zz -= zz >= Lz ? Lz : 0;
-                return (zz * Ly + yy) * Lx;
+                return R.coff + (zz * Ly + yy) * Lx;
             };
             int row = rowof(0);
             uint32_t nb0 = start[row + a0], ne0 = start[row + b0 + 1], nb1 = 0, ne1 = 0;
@@ -197,21 +229,12 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                 run(cb1, ce1);
             }
         } else {
-            for_stencil(P, start, cx, cy, cz, P->S, [&](uint32_t j) { body(j, ps[j]); });
+            for_stencil(R, start, cx, cy, cz, P->S, [&](uint32_t j) { body(j, ps[j]); });
         }
     }
-    float gx = (float)((double)ax * RD_FIX_INV);
-    float gy = (float)((double)ay * RD_FIX_INV);
-    float gz = (float)((double)az * RD_FIX_INV);
-    const int np1 = P->n_pot1;
-    const float prad = T.prad[ti];
-    for (int k = 0; k < np1; ++k) {
-        const RdPot1Dev& q = pot1[k];
-        if (!((q.mask >> ti) & 1u)) continue;
-        ext_grad<ENERGY>(q, prad, p.x, p.y, p.z, gx, gy, gz, &e);
-    }
+    float3 g = finish_grad<ENERGY>(P, T, pot1, p, ax, ay, az, &e);
     if (ENERGY) *energy = e;
-    return make_float3(gx, gy, gz);
+    return g;
 }
 
 __device__ __forceinline__ float wrap_coord(float x, float lo, float hi, float Lb)
@@ -221,19 +244,13 @@ __device__ __forceinline__ float wrap_coord(float x, float lo, float hi, float L
     return x;
 }
 
-__device__ __forceinline__ uint32_t cell_key(const RdDev* __restrict__ P, float x, float y, float z)
-{
-    int cx = cell_coord(x, P->lo[0], P->cell, P->L[0]);
-    int cy = cell_coord(y, P->lo[1], P->cell, P->L[1]);
-    int cz = cell_coord(z, P->lo[2], P->cell, P->L[2]);
-    return (uint32_t)((cz * P->L[1] + cy) * P->L[0] + cx);
-}
-
 // Euler-Maruyama for slot s (CudaSimulation.cu:1850-1858): noise, x += -a*dU
 // + b*xi, one periodic correction, the next cell key and its histogram entry.
-__device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, const RdTab& T,
+// The noise counter is (local index, step, replica stream); a caller-supplied
+// noise array is indexed by the global particle index.
+__device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
                                                 RdStepState* __restrict__ st, const float* __restrict__ noise,
-                                                int n, uint32_t s, float4 p, float3 g, float4* __restrict__ pn,
+                                                int ntot, uint32_t s, float4 p, float3 g, float4* __restrict__ pn,
                                                 uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                                                 uint32_t* __restrict__ count)
 {
@@ -242,12 +259,12 @@ __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, con
     const unsigned long long step = st->step;
     float3 z;
     if (st->noise_nsteps == 0) {
-        z = rd_normal3(P->seed, P->stream, step, idx);
+        z = rd_normal3(P->seed, R.stream, step, idx);
     } else {
         long long ls = (long long)(step - st->noise_step0);
         if (ls >= 0 && ls < st->noise_nsteps) {
-            size_t base = (size_t)ls * 3 * (size_t)n + idx;
-            z = make_float3(noise[base], noise[base + n], noise[base + 2 * (size_t)n]);
+            size_t base = (size_t)ls * 3 * (size_t)ntot + (size_t)R.off + idx;
+            z = make_float3(noise[base], noise[base + ntot], noise[base + 2 * (size_t)ntot]);
         } else {
             z = make_float3(0.0f, 0.0f, 0.0f);
             atomicOr(&st->err, 2u);
@@ -257,12 +274,21 @@ __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, con
     float x = (p.x - a * g.x) + bb * z.x;
     float y = (p.y - a * g.y) + bb * z.y;
     float zc = (p.z - a * g.z) + bb * z.z;
-    x = wrap_coord(x, P->lo[0], P->hi[0], P->Lb[0]);
-    y = wrap_coord(y, P->lo[1], P->hi[1], P->Lb[1]);
-    zc = wrap_coord(zc, P->lo[2], P->hi[2], P->Lb[2]);
+    x = wrap_coord(x, R.lo[0], R.hi[0], R.Lb[0]);
+    y = wrap_coord(y, R.lo[1], R.hi[1], R.Lb[1]);
+    zc = wrap_coord(zc, R.lo[2], R.hi[2], R.Lb[2]);
     if (!(fabsf(x) < 3.0e38f && fabsf(y) < 3.0e38f && fabsf(zc) < 3.0e38f)) atomicOr(&st->err, 1u);
     pn[s] = make_float4(x, y, zc, p.w);
-    uint32_t k = cell_key(P, x, y, zc);
+    uint32_t k = cell_key(R, x, y, zc);
     key[s] = k;
     rank[s] = atomicAdd(&count[k], 1u);
+}
+
+// the wave's work item: replica and slot range (waves never span replicas)
+__device__ __forceinline__ RdWave wave_item(const RdWave* __restrict__ waves, int nwaves, int w)
+{
+    RdWave it;
+    if (w < nwaves) it = waves[w];
+    else { it.rep = 0; it.s0 = 0; it.s1 = 0; it.pad = 0; }
+    return it;
 }

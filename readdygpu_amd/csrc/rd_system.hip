@@ -1,13 +1,16 @@
-// rd_system.hip -- the plain (single system) BD step behind the rd_* C ABI.
+// rd_system.hip -- the BD step behind the rd_* C ABI.
 //
-// Host side mirrors CudaSimulation (CudaSimulation.cuh:39-116): set-up,
-// step, transfers, RDF; the kernels are new gfx950 designs (see rd_kernels.h).
-#include "rd_kernels.h"
+// Host side mirrors CudaSimulation (CudaSimulation.cuh:39-116): set-up, step,
+// transfers, RDF.  A handle holds one system or a batch of independent
+// replicas; all replicas advance together through one set of kernels.
 #include "rd_host.h"
+#include "rd_kernels.h"
+#include "rd_tiled.h"
+#include "rd_wave.h"
 
 #include <algorithm>
-#include <cstddef>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -18,40 +21,82 @@
 // ===========================================================================
 
 // Pack unsorted host input into slot form and histogram the cells.
-__global__ __launch_bounds__(RD_BLOCK) void k_pack(const float* __restrict__ xyz, const int* __restrict__ type, int n,
-                                                   const RdDev* __restrict__ P, float4* __restrict__ pn,
+__global__ __launch_bounds__(RD_BLOCK) void k_pack(const float* __restrict__ xyz, const int* __restrict__ type,
+                                                   const int* __restrict__ rep_of, int n,
+                                                   const RdRep* __restrict__ reps, float4* __restrict__ pn,
                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                                                    uint32_t* __restrict__ count)
 {
     int i = blockIdx.x * RD_BLOCK + threadIdx.x;
     if (i >= n) return;
+    const RdRep R = reps[rep_of[i]];
     float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-    pn[i] = make_float4(x, y, z, w_pack((uint32_t)i, type[i]));
-    uint32_t k = cell_key(P, x, y, z);
+    pn[i] = make_float4(x, y, z, w_pack((uint32_t)(i - R.off), type[i]));
+    uint32_t k = cell_key(R, x, y, z);
     key[i] = k;
     rank[i] = atomicAdd(&count[k], 1u);
 }
 
-// One BD step for the particle in slot s (CudaSimulation.cu:2314-2465 fused):
-// gradient gather, Euler-Maruyama x += -a*dU + b*xi (CudaSimulation.cu:1852),
-// periodic wrap (:1857-1858), next cell key and its histogram.
+// wave index and lane of this thread; blocks of RD_BLOCK = 4 waves
+__device__ __forceinline__ int wave_id() { return xcd_block(blockIdx.x, gridDim.x) * (RD_BLOCK / 64) + (threadIdx.x >> 6); }
+
+// One BD step for the particle in slot s with the per-particle global gather
+// (CudaSimulation.cu:2314-2465 fused): gradient, Euler-Maruyama
+// (CudaSimulation.cu:1852), periodic wrap (:1857-1858), next cell key.
 __global__ __launch_bounds__(RD_BLOCK) void k_force_update(const float4* __restrict__ ps, float4* __restrict__ pn,
                                                            uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                                                            uint32_t* __restrict__ count,
                                                            const uint32_t* __restrict__ start,
                                                            const RdDev* __restrict__ P,
+                                                           const RdRep* __restrict__ reps,
+                                                           const RdWave* __restrict__ waves, int nwaves,
                                                            const RdPot1Dev* __restrict__ pot1,
                                                            RdStepState* __restrict__ st,
-                                                           const float* __restrict__ noise, int n)
+                                                           const float* __restrict__ noise, int ntot)
 {
     __shared__ RdTab T;
     load_tab(T, P);
-    int b = xcd_block(blockIdx.x, gridDim.x);
-    uint32_t s = (uint32_t)b * RD_BLOCK + threadIdx.x;
-    if (s >= (uint32_t)n) return;
+    const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(wave_id()));
+    const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
+    if (s >= (uint32_t)it.s1) return;
+    const RdRep R = reps[it.rep];
     float4 p = ps[s];
-    float3 g = gather_grad<false>(P, T, pot1, ps, start, s, p, nullptr);
-    em_update_store(P, T, st, noise, n, s, p, g, pn, key, rank, count);
+    float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr);
+    em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count);
+}
+
+// gradient + energy (FP64 accumulation) at the current positions
+__global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict__ ps,
+                                                        const uint32_t* __restrict__ start,
+                                                        const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
+                                                        const RdWave* __restrict__ waves, int nwaves,
+                                                        const RdPot1Dev* __restrict__ pot1,
+                                                        float* __restrict__ grad, double* __restrict__ etot)
+{
+    __shared__ double red[RD_BLOCK / 64];
+    __shared__ RdTab T;
+    load_tab(T, P);
+    const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(wave_id()));
+    const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
+    double e = 0.0;
+    if (s < (uint32_t)it.s1) {
+        const RdRep R = reps[it.rep];
+        float4 p = ps[s];
+        float ef = 0.0f;
+        float3 g = gather_grad<true>(P, R, T, pot1, ps, start, s, p, &ef);
+        size_t gi = (size_t)R.off + w_index(p.w);
+        grad[3 * gi] = g.x; grad[3 * gi + 1] = g.y; grad[3 * gi + 2] = g.z;
+        e = (double)ef;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < RD_BLOCK / 64; ++w) t += red[w];
+        atomicAdd(etot, t);
+    }
 }
 
 // ---- exclusive scan of count[ncell] -> start[ncell+1], zeroing count ----
@@ -107,40 +152,31 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_reduce(const uint32_t* __rest
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of the tile sums; writes start[ncell] = total
-__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* __restrict__ bsum, int nb,
-                                                    uint32_t* __restrict__ start, int ncell)
-{
-    __shared__ uint32_t lds[16];
-    __shared__ uint32_t carry;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        int i = base + threadIdx.x;
-        uint32_t v = i < nb ? bsum[i] : 0u;
-        uint32_t inc = wave_incl_scan(v);
-        if (lane == 63) lds[wid] = inc;
-        __syncthreads();
-        uint32_t off = carry, tot = 0;
-        for (int w = 0; w < 16; ++w) {
-            uint32_t t = lds[w];
-            if (w < wid) off += t;
-            tot += t;
-        }
-        if (i < nb) bsum[i] = off + inc - v;
-        __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) start[ncell] = carry;
-}
-
+// Each block adds up the tile sums before it (a few KB, L2-resident), scans
+// its own tile, writes start and zeroes count; the last block writes
+// start[ncell] = total.
 __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ count, int ncell,
                                                          const uint32_t* __restrict__ bsum,
                                                          uint32_t* __restrict__ start)
 {
     __shared__ uint32_t lds[RD_BLOCK / 64];
+    __shared__ uint32_t pre_sh;
+    {
+        uint32_t acc = 0;
+        for (int i = threadIdx.x; i < (int)blockIdx.x; i += RD_BLOCK) acc += bsum[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < RD_BLOCK / 64; ++w) t += lds[w];
+            pre_sh = t;
+            if (blockIdx.x == gridDim.x - 1) start[ncell] = t + bsum[blockIdx.x];
+        }
+        __syncthreads();
+    }
+    const uint32_t pre = pre_sh;
     size_t base = (size_t)blockIdx.x * RD_SCAN_TILE + (size_t)threadIdx.x * RD_SCAN_ITEMS;
     uint32_t c[RD_SCAN_ITEMS];
     bool full = base + RD_SCAN_ITEMS <= (size_t)ncell;
@@ -160,7 +196,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ 
     for (int k = 0; k < RD_SCAN_ITEMS; ++k) v += c[k];
     uint32_t ex;
     block_excl_scan(v, &ex, lds);
-    uint32_t run = bsum[blockIdx.x] + ex;
+    uint32_t run = pre + ex;
     if (full) {
         uint4* s4 = reinterpret_cast<uint4*>(start + base);
         uint4* z4 = reinterpret_cast<uint4*>(count + base);
@@ -193,102 +229,88 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__
     ps[dst] = pn[s];
 }
 
-// positions back to index order
-__global__ __launch_bounds__(RD_BLOCK) void k_unsort(const float4* __restrict__ ps, int n, float* __restrict__ xyz,
-                                                     int* __restrict__ type)
+// positions back to particle-index order
+__global__ __launch_bounds__(RD_BLOCK) void k_unsort(const float4* __restrict__ ps, const RdRep* __restrict__ reps,
+                                                     const RdWave* __restrict__ waves, int nwaves,
+                                                     float* __restrict__ xyz, int* __restrict__ type)
 {
-    int s = blockIdx.x * RD_BLOCK + threadIdx.x;
-    if (s >= n) return;
+    const RdWave it = wave_item(waves, nwaves, wave_id());
+    const int s = it.s0 + (threadIdx.x & 63);
+    if (s >= it.s1) return;
     float4 p = ps[s];
-    uint32_t i = w_index(p.w);
+    size_t i = (size_t)reps[it.rep].off + w_index(p.w);
     xyz[3 * i] = p.x; xyz[3 * i + 1] = p.y; xyz[3 * i + 2] = p.z;
     type[i] = w_type(p.w);
 }
 
-// reference cell index per particle index
-__global__ __launch_bounds__(RD_BLOCK) void k_cell_ids(const float4* __restrict__ ps, int n,
-                                                       const RdDev* __restrict__ P, int* __restrict__ field)
+// reference cell index (replica-local) per particle
+__global__ __launch_bounds__(RD_BLOCK) void k_cell_ids(const float4* __restrict__ ps, const RdRep* __restrict__ reps,
+                                                       const RdWave* __restrict__ waves, int nwaves,
+                                                       int* __restrict__ field)
 {
-    int s = blockIdx.x * RD_BLOCK + threadIdx.x;
-    if (s >= n) return;
+    const RdWave it = wave_item(waves, nwaves, wave_id());
+    const int s = it.s0 + (threadIdx.x & 63);
+    if (s >= it.s1) return;
+    const RdRep R = reps[it.rep];
     float4 p = ps[s];
-    field[w_index(p.w)] = (int)cell_key(P, p.x, p.y, p.z);
+    field[(size_t)R.off + w_index(p.w)] = (int)cell_key(R, p.x, p.y, p.z) - R.coff;
 }
 
-// gradient + energy (FP64 accumulation) at the current positions
-__global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict__ ps,
-                                                        const uint32_t* __restrict__ start,
-                                                        const RdDev* __restrict__ P,
-                                                        const RdPot1Dev* __restrict__ pot1, int n,
-                                                        float* __restrict__ grad, double* __restrict__ etot)
-{
-    __shared__ double red[RD_BLOCK / 64];
-    __shared__ RdTab T;
-    load_tab(T, P);
-    uint32_t s = blockIdx.x * RD_BLOCK + threadIdx.x;
-    double e = 0.0;
-    if (s < (uint32_t)n) {
-        float4 p = ps[s];
-        float ef = 0.0f;
-        float3 g = gather_grad<true>(P, T, pot1, ps, start, s, p, &ef);
-        uint32_t i = w_index(p.w);
-        grad[3 * i] = g.x; grad[3 * i + 1] = g.y; grad[3 * i + 2] = g.z;
-        e = (double)ef;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < RD_BLOCK / 64; ++w) t += red[w];
-        atomicAdd(etot, t);
-    }
-}
-
-// noise arrays out[(s*3+d)*n + i]
-__global__ __launch_bounds__(RD_BLOCK) void k_noise_fill(const RdDev* __restrict__ P, unsigned long long step0,
-                                                         int nsteps, int n, float* __restrict__ out)
+// noise arrays out[(s*3+d)*ntot + i], counter (local index, step, replica stream)
+__global__ __launch_bounds__(RD_BLOCK) void k_noise_fill(const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
+                                                         int nrep, unsigned long long step0, int nsteps, int ntot,
+                                                         float* __restrict__ out)
 {
     size_t g = (size_t)blockIdx.x * RD_BLOCK + threadIdx.x;
-    size_t total = (size_t)nsteps * (size_t)n;
+    size_t total = (size_t)nsteps * (size_t)ntot;
     if (g >= total) return;
-    int s = (int)(g / (size_t)n);
-    int i = (int)(g - (size_t)s * n);
-    float3 z = rd_normal3(P->seed, P->stream, step0 + (unsigned long long)s, (uint32_t)i);
-    size_t base = (size_t)s * 3 * (size_t)n + i;
-    out[base] = z.x; out[base + n] = z.y; out[base + 2 * (size_t)n] = z.z;
+    int s = (int)(g / (size_t)ntot);
+    int i = (int)(g - (size_t)s * ntot);
+    int lo = 0, hi = nrep - 1;          // last replica with off <= i
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (reps[mid].off <= i) lo = mid; else hi = mid - 1;
+    }
+    const RdRep& R = reps[lo];
+    float3 z = rd_normal3(P->seed, R.stream, step0 + (unsigned long long)s, (uint32_t)(i - R.off));
+    size_t base = (size_t)s * 3 * (size_t)ntot + i;
+    out[base] = z.x; out[base + ntot] = z.y; out[base + 2 * (size_t)ntot] = z.z;
 }
 
-// RDF histogram, privatised in LDS (CudaSimulation.cu:1784-1827 counts every
-// ordered pair once per side; here each slot counts its own partners).
+// RDF histogram, privatised per wave in LDS (CudaSimulation.cu:1784-1827 counts
+// every ordered pair once per side; here each slot counts its own partners).
+// counts[((rep*nt + ti)*nt + tj)*nbins + bin]
 template <bool LDS>
 __global__ __launch_bounds__(RD_BLOCK) void k_rdf(const float4* __restrict__ ps, const uint32_t* __restrict__ start,
-                                                  const RdDev* __restrict__ P, int S, float rmax, int nbins, int n,
-                                                  unsigned long long* __restrict__ counts)
+                                                  const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
+                                                  const RdWave* __restrict__ waves, int nwaves, int S, float rmax,
+                                                  int nbins, unsigned long long* __restrict__ counts)
 {
-    extern __shared__ uint32_t hist[];
+    extern __shared__ uint32_t hist_all[];
     const int nt = P->nt;
     const int nh = nt * nt * nbins;
+    const int wv = threadIdx.x >> 6;
+    uint32_t* hist = hist_all + (size_t)wv * nh;
     if (LDS) {
-        for (int k = threadIdx.x; k < nh; k += RD_BLOCK) hist[k] = 0u;
+        for (int k = threadIdx.x; k < nh * (RD_BLOCK / 64); k += RD_BLOCK) hist_all[k] = 0u;
         __syncthreads();
     }
-    int b = xcd_block(blockIdx.x, gridDim.x);
-    uint32_t s = (uint32_t)b * RD_BLOCK + threadIdx.x;
-    if (s < (uint32_t)n) {
+    const RdWave it = wave_item(waves, nwaves, wave_id());
+    const int s = it.s0 + (threadIdx.x & 63);
+    const RdRep R = reps[it.rep];
+    unsigned long long* cnt = counts + (size_t)it.rep * nh;
+    if (s < it.s1) {
         float4 p = ps[s];
         int ti = w_type(p.w);
-        int cx = cell_coord(p.x, P->lo[0], P->cell, P->L[0]);
-        int cy = cell_coord(p.y, P->lo[1], P->cell, P->L[1]);
-        int cz = cell_coord(p.z, P->lo[2], P->cell, P->L[2]);
+        int cx, cy, cz;
+        cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
         const float fnb = (float)nbins;
-        for_stencil(P, start, cx, cy, cz, S, [&](uint32_t j) {
-            if (j == s) return;
+        for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
+            if ((int)j == s) return;
             float4 q = ps[j];
-            float dx = min_image(p.x - q.x, P->Lb[0], P->half[0]);
-            float dy = min_image(p.y - q.y, P->Lb[1], P->half[1]);
-            float dz = min_image(p.z - q.z, P->Lb[2], P->half[2]);
+            float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
+            float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
+            float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
             float d2 = dx * dx;
             d2 = d2 + dy * dy;
             d2 = d2 + dz * dz;
@@ -298,47 +320,51 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf(const float4* __restrict__ ps,
                 if (bin > nbins - 1) bin = nbins - 1;
                 int h = (ti * nt + w_type(q.w)) * nbins + bin;
                 if (LDS) atomicAdd(&hist[h], 1u);
-                else atomicAdd(&counts[h], 1ull);
+                else atomicAdd(&cnt[h], 1ull);
             }
         });
     }
     if (LDS) {
         __syncthreads();
-        for (int k = threadIdx.x; k < nh; k += RD_BLOCK) {
-            uint32_t v = hist[k];
-            if (v) atomicAdd(&counts[k], (unsigned long long)v);
+        if (it.s1 > it.s0) {
+            for (int k = threadIdx.x & 63; k < nh; k += 64) {
+                uint32_t v = hist[k];
+                if (v) atomicAdd(&cnt[k], (unsigned long long)v);
+            }
         }
     }
 }
 
-// pairs (i<j by index) with d2 < thr2[ti][tj]; wave-aggregated append
+// pairs (i<j by index) with d2 < thr2[ti][tj]; global particle indices
 __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ ps, const uint32_t* __restrict__ start,
-                                                    const RdDev* __restrict__ P, const float* __restrict__ thr2,
-                                                    int S, int n, uint2* __restrict__ out, unsigned long long cap,
-                                                    unsigned long long* __restrict__ counter)
+                                                    const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
+                                                    const RdWave* __restrict__ waves, int nwaves,
+                                                    const float* __restrict__ thr2, int S, uint2* __restrict__ out,
+                                                    unsigned long long cap, unsigned long long* __restrict__ counter)
 {
-    uint32_t s = blockIdx.x * RD_BLOCK + threadIdx.x;
-    if (s >= (uint32_t)n) return;
+    const RdWave it = wave_item(waves, nwaves, wave_id());
+    const int s = it.s0 + (threadIdx.x & 63);
+    if (s >= it.s1) return;
+    const RdRep R = reps[it.rep];
     float4 p = ps[s];
     const int nt = P->nt;
     int ti = w_type(p.w);
     uint32_t ii = w_index(p.w);
-    int cx = cell_coord(p.x, P->lo[0], P->cell, P->L[0]);
-    int cy = cell_coord(p.y, P->lo[1], P->cell, P->L[1]);
-    int cz = cell_coord(p.z, P->lo[2], P->cell, P->L[2]);
-    for_stencil(P, start, cx, cy, cz, S, [&](uint32_t j) {
+    int cx, cy, cz;
+    cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
+    for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
         float4 q = ps[j];
         uint32_t jj = w_index(q.w);
         if (jj <= ii) return;
-        float dx = min_image(p.x - q.x, P->Lb[0], P->half[0]);
-        float dy = min_image(p.y - q.y, P->Lb[1], P->half[1]);
-        float dz = min_image(p.z - q.z, P->Lb[2], P->half[2]);
+        float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
+        float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
+        float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
         float d2 = dx * dx;
         d2 = d2 + dy * dy;
         d2 = d2 + dz * dz;
         if (d2 < thr2[ti * nt + w_type(q.w)]) {
             unsigned long long slot = atomicAdd(counter, 1ull);
-            if (slot < cap) out[slot] = make_uint2(ii, jj);
+            if (slot < cap) out[slot] = make_uint2((uint32_t)R.off + ii, (uint32_t)R.off + jj);
         }
     });
 }
@@ -362,11 +388,20 @@ static int set_err(rd_system* h, int code, const std::string& m)
 
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
+static void drop_graph(rd_system* h)
+{
+    for (auto& kv : h->graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    h->graphs.clear();
+}
+
 static void free_dev(rd_system* h)
 {
     void* ptrs[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_bsum,
                      h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_pairs, h->d_pcount,
-                     h->d_xyz_io, h->d_type_io, h->d_etot };
+                     h->d_xyz_io, h->d_type_io, h->d_etot, h->d_reps, h->d_waves, h->d_rep_of };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->d_ps = h->d_pn = nullptr;
@@ -374,13 +409,8 @@ static void free_dev(rd_system* h)
     h->d_P = nullptr; h->d_pot1 = nullptr; h->d_st = nullptr; h->d_noise = nullptr;
     h->d_rdf = nullptr; h->d_pairs = nullptr; h->d_pcount = nullptr;
     h->d_xyz_io = nullptr; h->d_type_io = nullptr; h->d_etot = nullptr;
-    h->cap = 0; h->ncell_alloc = 0; h->pairs_cap = 0; h->noise_cap = 0; h->io_cap = 0;
-}
-
-static void drop_graph(rd_system* h)
-{
-    if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
-    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    h->d_reps = nullptr; h->d_waves = nullptr; h->d_rep_of = nullptr;
+    h->cap = 0; h->ncell_alloc = 0; h->pairs_cap = 0; h->noise_cap = 0; h->nrep_alloc = 0; h->nwave_alloc = 0;
 }
 
 extern "C" int rd_create(const rd_params* prm, rd_system** out)
@@ -413,8 +443,6 @@ extern "C" void rd_destroy(rd_system* h)
     drop_graph(h);
     free_dev(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
-    if (h->ev0) (void)hipEventDestroy(h->ev0);
-    if (h->ev1) (void)hipEventDestroy(h->ev1);
     delete h;
 }
 
@@ -502,7 +530,9 @@ double rd_host_r0(const rd_system* h, int i, int j)
     return ri + rj;
 }
 
-// Build the device constant block (CudaSimulation.cu:237-391 tables).
+// Build the device constant block (CudaSimulation.cu:237-391 tables) and the
+// replica geometry: per replica L_d = (int)((hi-lo)/cell) in FP32
+// (CudaSimulation.cu:524-528), cells of all replicas concatenated.
 static int build_tables(rd_system* h)
 {
     RdDev& P = h->P;
@@ -510,12 +540,6 @@ static int build_tables(rd_system* h)
     const rd_params& g = h->prm;
     int nt = h->nt;
     P.nt = nt;
-    for (int d = 0; d < 3; ++d) {
-        P.lo[d] = (float)g.lattice_bounds[2 * d];
-        P.hi[d] = (float)g.lattice_bounds[2 * d + 1];
-        P.Lb[d] = P.hi[d] - P.lo[d];
-        P.half[d] = P.Lb[d] / 2.0f;
-    }
     double maxr = 0.0, maxrx = 0.0;
     for (int i = 0; i < nt; ++i)
         for (int j = 0; j < nt; ++j) {
@@ -535,17 +559,35 @@ static int build_tables(rd_system* h)
     h->max_rxn = maxrx;
     double mr = std::max(maxr, maxrx);
     double cell = g.cell_size > 0 ? g.cell_size : (mr > 0 ? 1.01 * mr : g.max_cutoff);
-    P.cell = (float)cell;
     h->cell = cell;
-    long nc = 1;
-    for (int d = 0; d < 3; ++d) {
-        int l = (int)(P.Lb[d] / P.cell);
-        P.L[d] = l < 1 ? 1 : l;
-        nc *= P.L[d];
-    }
-    if (nc > (1L << 28)) return set_err(h, RD_ERANGE, "too many cells; raise cell_size");
-    h->ncell = (int)nc;
     P.S = std::max(1, (int)std::ceil(maxr * 1.0001 / cell));
+    long coff = 0;
+    int off = 0;
+    h->rep_dev.resize(h->reps.size());
+    for (size_t r = 0; r < h->reps.size(); ++r) {
+        const rd_system::Rep& A = h->reps[r];
+        RdRep& R = h->rep_dev[r];
+        memset(&R, 0, sizeof(R));
+        R.cell = (float)cell;
+        long nc = 1;
+        for (int d = 0; d < 3; ++d) {
+            R.lo[d] = (float)A.lo[d];
+            R.hi[d] = (float)A.hi[d];
+            R.Lb[d] = R.hi[d] - R.lo[d];
+            R.half[d] = R.Lb[d] / 2.0f;
+            int l = (int)(R.Lb[d] / R.cell);
+            R.L[d] = l < 1 ? 1 : l;
+            nc *= R.L[d];
+        }
+        if (coff + nc > (1L << 28)) return set_err(h, RD_ERANGE, "too many cells; raise cell_size");
+        R.coff = (int)coff;
+        R.off = off;
+        R.n = A.n;
+        R.stream = A.stream;
+        coff += nc;
+        off += A.n;
+    }
+    h->ncell = (int)std::max(coff, 1L);
     P.n_pot2 = (int)h->pot2.size();
     for (int p = 0; p < P.n_pot2; ++p) {
         const rd_pot2& q = h->pot2[p];
@@ -560,7 +602,6 @@ static int build_tables(rd_system* h)
         P.prad[t] = (float)h->rad[t];
     }
     P.seed = g.seed;
-    P.stream = g.stream;
     P.n_pot1 = (int)h->pot1.size();
     h->pot1dev.resize(h->pot1.size());
     for (size_t k = 0; k < h->pot1.size(); ++k) {
@@ -579,6 +620,31 @@ static int build_tables(rd_system* h)
         d.R = q.radius;
         d.mask = q.type_mask;
     }
+    // waves: 64 consecutive slots of one replica each
+    h->wave_tab.clear();
+    for (size_t r = 0; r < h->rep_dev.size(); ++r) {
+        const RdRep& R = h->rep_dev[r];
+        for (int s = 0; s < R.n; s += 64) {
+            RdWave w;
+            w.rep = (int)r; w.s0 = R.off + s; w.s1 = R.off + std::min(R.n, s + 64); w.pad = 0;
+            h->wave_tab.push_back(w);
+        }
+    }
+    h->nwaves = (int)h->wave_tab.size();
+    return RD_OK;
+}
+
+static int validate_particles(rd_system* h, int n, const float* xyz, const int* type, const double lo[3],
+                              const double hi[3], int base)
+{
+    for (int i = 0; i < n; ++i) {
+        if (type[i] < 0 || type[i] >= h->nt) return set_err(h, RD_EINVAL, "particle type out of range");
+        for (int d = 0; d < 3; ++d) {
+            float x = xyz[3 * i + d];
+            if (!(x >= (float)lo[d] && x <= (float)hi[d]))
+                return set_err(h, RD_ERANGE, "particle is out of the box: " + std::to_string(base + i));
+        }
+    }
     return RD_OK;
 }
 
@@ -587,18 +653,57 @@ extern "C" int rd_set_particles(rd_system* h, int n, const float* xyz, const int
     if (!h) return RD_EINVAL;
     if (n < 0 || n > RD_MAX_PARTICLES || (n > 0 && (!xyz || !type))) return set_err(h, RD_EINVAL, "bad particle arrays");
     if (h->nt == 0) return set_err(h, RD_ESTATE, "set particle types first");
-    for (int i = 0; i < n; ++i) {
-        if (type[i] < 0 || type[i] >= h->nt) return set_err(h, RD_EINVAL, "particle type out of range");
-        for (int d = 0; d < 3; ++d) {
-            float x = xyz[3 * i + d];
-            if (!(x >= (float)h->prm.lattice_bounds[2 * d] && x <= (float)h->prm.lattice_bounds[2 * d + 1]))
-                return set_err(h, RD_ERANGE, "particle is out of the box: " + std::to_string(i));
-        }
+    rd_system::Rep A;
+    for (int d = 0; d < 3; ++d) {
+        A.lo[d] = h->prm.lattice_bounds[2 * d];
+        A.hi[d] = h->prm.lattice_bounds[2 * d + 1];
     }
+    A.stream = h->prm.stream;
+    A.n = n;
+    int rc = validate_particles(h, n, xyz, type, A.lo, A.hi, 0);
+    if (rc) return rc;
+    h->reps.assign(1, A);
     h->n = n;
     h->hxyz.assign(xyz, xyz + 3 * (size_t)n);
     h->htype.assign(type, type + n);
     h->particles_dirty = true;
+    h->dirty = true;
+    return RD_OK;
+}
+
+extern "C" int rd_set_replicas(rd_system* h, int nrep, const double* bounds, const uint32_t* streams,
+                               const int* counts, const float* xyz, const int* type)
+{
+    if (!h || nrep < 1 || !bounds || !counts) return RD_EINVAL;
+    if (h->nt == 0) return set_err(h, RD_ESTATE, "set particle types first");
+    long total = 0;
+    for (int r = 0; r < nrep; ++r) {
+        if (counts[r] < 0) return set_err(h, RD_EINVAL, "negative replica size");
+        total += counts[r];
+    }
+    if (total > RD_MAX_PARTICLES) return set_err(h, RD_EINVAL, "too many particles");
+    if (total > 0 && (!xyz || !type)) return set_err(h, RD_EINVAL, "bad particle arrays");
+    std::vector<rd_system::Rep> reps(nrep);
+    long base = 0;
+    for (int r = 0; r < nrep; ++r) {
+        rd_system::Rep& A = reps[r];
+        for (int d = 0; d < 3; ++d) {
+            A.lo[d] = bounds[6 * r + 2 * d];
+            A.hi[d] = bounds[6 * r + 2 * d + 1];
+            if (!(A.hi[d] > A.lo[d])) return set_err(h, RD_EINVAL, "bad replica box");
+        }
+        A.stream = streams ? streams[r] : (uint32_t)r;
+        A.n = counts[r];
+        int rc = validate_particles(h, A.n, xyz + 3 * base, type + base, A.lo, A.hi, (int)base);
+        if (rc) return rc;
+        base += A.n;
+    }
+    h->reps = reps;
+    h->n = (int)total;
+    h->hxyz.assign(xyz, xyz + 3 * (size_t)total);
+    h->htype.assign(type, type + total);
+    h->particles_dirty = true;
+    h->dirty = true;
     return RD_OK;
 }
 
@@ -608,7 +713,7 @@ static int ensure_capacity(rd_system* h)
     if (n > h->cap) {
         drop_graph(h);
         int cap = std::max(n, h->cap + h->cap / 2);
-        void* ps[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_xyz_io, h->d_type_io };
+        void* ps[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_xyz_io, h->d_type_io, h->d_rep_of };
         for (void* p : ps)
             if (p) (void)hipFree(p);
         HIPCK(h, hipMalloc(&h->d_ps, sizeof(float4) * cap));
@@ -617,6 +722,7 @@ static int ensure_capacity(rd_system* h)
         HIPCK(h, hipMalloc(&h->d_rank, sizeof(uint32_t) * cap));
         HIPCK(h, hipMalloc(&h->d_xyz_io, sizeof(float) * 3 * cap));
         HIPCK(h, hipMalloc(&h->d_type_io, sizeof(int) * cap));
+        HIPCK(h, hipMalloc(&h->d_rep_of, sizeof(int) * cap));
         h->cap = cap;
     }
     if (h->ncell != h->ncell_alloc) {
@@ -631,6 +737,18 @@ static int ensure_capacity(rd_system* h)
         HIPCK(h, hipMalloc(&h->d_bsum, sizeof(uint32_t) * (h->nscan + 16)));
         HIPCK(h, hipMemsetAsync(h->d_count, 0, sizeof(uint32_t) * (nc + 16), h->stream));
         h->ncell_alloc = h->ncell;
+    }
+    if ((int)h->rep_dev.size() > h->nrep_alloc) {
+        drop_graph(h);
+        if (h->d_reps) (void)hipFree(h->d_reps);
+        HIPCK(h, hipMalloc(&h->d_reps, sizeof(RdRep) * h->rep_dev.size()));
+        h->nrep_alloc = (int)h->rep_dev.size();
+    }
+    if (std::max(h->nwaves, 1) > h->nwave_alloc) {
+        drop_graph(h);
+        if (h->d_waves) (void)hipFree(h->d_waves);
+        HIPCK(h, hipMalloc(&h->d_waves, sizeof(RdWave) * std::max(h->nwaves, 1)));
+        h->nwave_alloc = std::max(h->nwaves, 1);
     }
     if (!h->d_P) HIPCK(h, hipMalloc(&h->d_P, sizeof(RdDev)));
     if (!h->d_st) {
@@ -649,25 +767,25 @@ static void launch_sort(rd_system* h, int advance)
 {
     int nc = h->ncell;
     hipLaunchKernelGGL(k_scan_reduce, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, h->stream, h->d_bsum, h->nscan, h->d_start, nc);
     hipLaunchKernelGGL(k_scan_apply, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum,
                        h->d_start);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(h->n, 1), RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_pn,
                        h->d_key, h->d_rank, h->d_start, h->d_ps, h->n, h->d_st, advance);
 }
 
-void rd_host_launch_force(rd_system* h);
+static int wave_blocks(const rd_system* h) { return std::max(1, nblk(h->nwaves, RD_BLOCK / 64)); }
 
-// Tile shape for the LDS-tiled kernel: home particles per tile near
-// fill x block, staged capacity with headroom for density fluctuations,
-// LDS <= 64 KB per workgroup.
+// Tile shape for the LDS-tiled kernel (single replica): home particles per
+// tile near fill x block, staged capacity with headroom for density
+// fluctuations, LDS <= 64 KB per workgroup.
 static void choose_tiles(rd_system* h)
 {
-    const RdDev& P = h->P;
-    const int Lx = P.L[0], Ly = P.L[1], Lz = P.L[2];
     h->tiled = false;
     h->tile_block = h->lc.block > 0 ? h->lc.block : 256;
-    if (h->lc.kernel == RD_KERNEL_GATHER) return;
+    if (h->lc.kernel != RD_KERNEL_TILED || h->rep_dev.size() != 1) return;
+    const RdRep& R0 = h->rep_dev[0];
+    const RdDev& P = h->P;
+    const int Lx = R0.L[0], Ly = R0.L[1], Lz = R0.L[2];
     if (P.S != 1 || Lx < 3 || Ly < 3 || Lz < 3 || P.n_pot2 == 0 || h->n == 0) return;
     const double rc = (double)h->n / ((double)Lx * Ly * Lz);
     const double fill = h->lc.fill > 0 ? h->lc.fill : 0.6;
@@ -693,7 +811,7 @@ static void choose_tiles(rd_system* h)
         size_t lds = rd_tile_lds_bytes(G);
         if (lds <= 64 * 1024) { h->tile_lds = lds; break; }
         if (G.TY > 1) { G.TY--; continue; }
-        if (G.nseg == 1 || G.TW <= 1) return;       // cannot fit: gather kernel
+        if (G.nseg == 1 || G.TW <= 1) return;       // cannot fit: another kernel
         G.TW = std::max(1, G.TW / 2);
         G.nseg = (Lx + G.TW - 1) / G.TW;
         G.W = G.TW + 2;
@@ -710,10 +828,10 @@ static void launch_tiled(rd_system* h, float* grad, double* etot)
     const RdTileGeom& G = h->tg;
     dim3 grid(G.ntiles), blk(h->tile_block);
     size_t lds = h->tile_lds;
-#define RD_TL(B)                                                                                         \
-    hipLaunchKernelGGL((k_force_tiled<B, MODE>), grid, blk, lds, h->stream, h->d_ps, h->d_pn, h->d_key, \
-                       h->d_rank, h->d_count, h->d_start, h->d_P, h->d_pot1, h->d_st, h->d_noise, h->n, G, \
-                       grad, etot)
+#define RD_TL(B)                                                                                           \
+    hipLaunchKernelGGL((k_force_tiled<B, MODE>), grid, blk, lds, h->stream, h->d_ps, h->d_pn, h->d_key,   \
+                       h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps, h->d_pot1, h->d_st, h->d_noise, \
+                       h->n, G, grad, etot)
     switch (h->tile_block) {
     case 64: RD_TL(64); break;
     case 128: RD_TL(128); break;
@@ -721,6 +839,25 @@ static void launch_tiled(rd_system* h, float* grad, double* etot)
     default: RD_TL(256); break;
     }
 #undef RD_TL
+}
+
+void rd_host_launch_force(rd_system* h)
+{
+    if (h->n <= 0) return;
+    if (h->kern == RD_KERNEL_TILED) {
+        launch_tiled<RD_MODE_UPDATE>(h, nullptr, nullptr);
+        return;
+    }
+    if (h->kern == RD_KERNEL_WAVE) {
+        hipLaunchKernelGGL(k_force_wave<RD_MODE_UPDATE>, dim3(nblk(h->nwaves, RD_WAVE_WAVES)),
+                           dim3(64 * RD_WAVE_WAVES), 0, h->stream, h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count,
+                           h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st, h->d_noise, h->n,
+                           nullptr, nullptr);
+        return;
+    }
+    hipLaunchKernelGGL(k_force_update, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_pn, h->d_key,
+                       h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st,
+                       h->d_noise, h->n);
 }
 
 static void launch_step(rd_system* h)
@@ -737,7 +874,13 @@ static int upload(rd_system* h)
     rc = ensure_capacity(h);
     if (rc) return rc;
     choose_tiles(h);
+    h->kern = h->tiled ? RD_KERNEL_TILED : (h->lc.kernel == RD_KERNEL_GATHER ? RD_KERNEL_GATHER : RD_KERNEL_WAVE);
     HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->d_reps, h->rep_dev.data(), sizeof(RdRep) * h->rep_dev.size(), hipMemcpyHostToDevice,
+                            h->stream));
+    if (h->nwaves > 0)
+        HIPCK(h, hipMemcpyAsync(h->d_waves, h->wave_tab.data(), sizeof(RdWave) * h->nwaves, hipMemcpyHostToDevice,
+                                h->stream));
     if (h->noise_n >= 0 && h->noise_n != h->n) {
         // the noise array was laid out for another particle count: back to the generator
         RdStepState st;
@@ -754,10 +897,14 @@ static int upload(rd_system* h)
                                 hipMemcpyHostToDevice, h->stream));
     }
     if (h->n > 0) {
+        h->hrep_of.resize(h->n);
+        for (size_t r = 0; r < h->rep_dev.size(); ++r)
+            for (int i = 0; i < h->rep_dev[r].n; ++i) h->hrep_of[h->rep_dev[r].off + i] = (int)r;
         HIPCK(h, hipMemcpyAsync(h->d_xyz_io, h->hxyz.data(), sizeof(float) * 3 * h->n, hipMemcpyHostToDevice, h->stream));
         HIPCK(h, hipMemcpyAsync(h->d_type_io, h->htype.data(), sizeof(int) * h->n, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->d_rep_of, h->hrep_of.data(), sizeof(int) * h->n, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io, h->d_type_io,
-                           h->n, h->d_P, h->d_pn, h->d_key, h->d_rank, h->d_count);
+                           h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count);
     }
     launch_sort(h, 0);
     HIPCK(h, hipGetLastError());
@@ -772,6 +919,7 @@ extern "C" int rd_initialize(rd_system* h)
 {
     if (!h) return RD_EINVAL;
     if (h->nt == 0) return set_err(h, RD_ESTATE, "no particle types");
+    if (h->reps.empty()) return set_err(h, RD_ESTATE, "no particles set");
     HIPCK(h, hipSetDevice(h->dev));
     int rc = upload(h);
     if (rc) return rc;
@@ -791,21 +939,50 @@ static int refresh(rd_system* h)
     return RD_OK;
 }
 
+// One hipGraph per step count: K steps captured once, launched as one graph.
+static int step_graph(rd_system* h, int k, hipGraphExec_t* exec)
+{
+    auto it = h->graphs.find(k);
+    if (it != h->graphs.end()) { *exec = it->second.exec; return RD_OK; }
+    rd_system::Graph gr;
+    HIPCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < k; ++i) launch_step(h);
+    hipError_t e = hipStreamEndCapture(h->stream, &gr.graph);
+    if (e != hipSuccess) return set_err(h, RD_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(gr.graph);
+        return set_err(h, RD_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    h->graphs[k] = gr;
+    *exec = gr.exec;
+    return RD_OK;
+}
+
+// launch nsteps steps as graphs of up to RD_GRAPH_STEPS steps
+static int launch_steps(rd_system* h, int nsteps)
+{
+    const int chunk = 256;
+    while (nsteps > 0) {
+        int k = nsteps >= chunk ? chunk : (nsteps >= 16 ? 16 : 1);
+        hipGraphExec_t ex;
+        int rc = step_graph(h, k, &ex);
+        if (rc) return rc;
+        HIPCK(h, hipGraphLaunch(ex, h->stream));
+        nsteps -= k;
+        h->step += (uint64_t)k;
+    }
+    return RD_OK;
+}
+
 extern "C" int rd_simulate(rd_system* h, int nsteps)
 {
     if (!h || nsteps < 0) return RD_EINVAL;
     int rc = refresh(h);
     if (rc) return rc;
     if (nsteps == 0) return RD_OK;
-    if (!h->graph_exec) {
-        HIPCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        launch_step(h);
-        hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
-        if (e != hipSuccess) return set_err(h, RD_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-        HIPCK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
-    }
-    for (int i = 0; i < nsteps; ++i) HIPCK(h, hipGraphLaunch(h->graph_exec, h->stream));
-    h->step += (uint64_t)nsteps;
+    rc = launch_steps(h, nsteps);
+    if (rc) return rc;
     RdStepState st;
     HIPCK(h, hipMemcpyAsync(&st, h->d_st, sizeof(st), hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
@@ -816,6 +993,8 @@ extern "C" int rd_simulate(rd_system* h, int nsteps)
 
 // pieces of the step for the measurement entry points (rd_bench.hip)
 int rd_host_refresh(rd_system* h) { return refresh(h); }
+void rd_host_launch_sort(rd_system* h, int advance) { launch_sort(h, advance); }
+int rd_host_step_graph(rd_system* h, int k, hipGraphExec_t* exec) { return step_graph(h, k, exec); }
 
 // reset the global step counter (noise counter), e.g. after autotuning
 int rd_host_set_step(rd_system* h, uint64_t step)
@@ -830,22 +1009,21 @@ int rd_host_set_step(rd_system* h, uint64_t step)
     h->step = step;
     return RD_OK;
 }
-void rd_host_launch_sort(rd_system* h, int advance) { launch_sort(h, advance); }
-void rd_host_launch_force(rd_system* h)
+
+// replace positions and types, keeping the replica layout (autotuner)
+int rd_host_set_state(rd_system* h, const std::vector<float>& xyz, const std::vector<int>& type)
 {
-    if (h->n <= 0) return;
-    if (h->tiled) {
-        launch_tiled<RD_TILE_UPDATE>(h, nullptr, nullptr);
-        return;
-    }
-    hipLaunchKernelGGL(k_force_update, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_pn,
-                       h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P, h->d_pot1, h->d_st, h->d_noise, h->n);
+    if (xyz.size() != 3 * (size_t)h->n || type.size() != (size_t)h->n) return set_err(h, RD_EINVAL, "state size");
+    h->hxyz = xyz;
+    h->htype = type;
+    h->particles_dirty = true;
+    return RD_OK;
 }
 
 extern "C" int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg)
 {
     if (!h || !cfg) return RD_EINVAL;
-    if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_TILED) return set_err(h, RD_EINVAL, "bad kernel");
+    if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_WAVE) return set_err(h, RD_EINVAL, "bad kernel");
     if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256 && cfg->block != 512)
         return set_err(h, RD_EINVAL, "block must be 64, 128, 256 or 512");
     if (cfg->rows_per_tile < 0 || cfg->rows_per_tile > 8 || cfg->fill < 0)
@@ -859,7 +1037,7 @@ extern "C" int rd_get_launch_config(rd_system* h, rd_launch_config* cfg)
 {
     if (!h || !cfg) return RD_EINVAL;
     *cfg = h->lc;
-    cfg->used_kernel = h->tiled ? RD_KERNEL_TILED : RD_KERNEL_GATHER;
+    cfg->used_kernel = h->kern;
     cfg->used_rows = h->tiled ? h->tg.TY : 0;
     cfg->used_segments = h->tiled ? h->tg.nseg : 0;
     cfg->used_cap = h->tiled ? h->tg.cap : 0;
@@ -879,8 +1057,8 @@ extern "C" int rd_copy_pos_from_device(rd_system* h, float* xyz, int* type)
     }
     HIPCK(h, hipSetDevice(h->dev));
     if (h->n > 0) {
-        hipLaunchKernelGGL(k_unsort, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->n,
-                           h->d_xyz_io, h->d_type_io);
+        hipLaunchKernelGGL(k_unsort, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_reps,
+                           h->d_waves, h->nwaves, h->d_xyz_io, h->d_type_io);
         HIPCK(h, hipMemcpyAsync(h->hxyz.data(), h->d_xyz_io, sizeof(float) * 3 * h->n, hipMemcpyDeviceToHost, h->stream));
         HIPCK(h, hipMemcpyAsync(h->htype.data(), h->d_type_io, sizeof(int) * h->n, hipMemcpyDeviceToHost, h->stream));
     }
@@ -896,8 +1074,8 @@ extern "C" int rd_cell_ids(rd_system* h, int* field)
     int rc = refresh(h);
     if (rc) return rc;
     if (h->n == 0) return RD_OK;
-    hipLaunchKernelGGL(k_cell_ids, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->n, h->d_P,
-                       h->d_type_io);
+    hipLaunchKernelGGL(k_cell_ids, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_reps, h->d_waves,
+                       h->nwaves, h->d_type_io);
     HIPCK(h, hipGetLastError());
     HIPCK(h, hipMemcpyAsync(field, h->d_type_io, sizeof(int) * h->n, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
@@ -911,11 +1089,16 @@ extern "C" int rd_gradients(rd_system* h, float* grad, double* energy)
     if (rc) return rc;
     if (h->n == 0) { if (energy) *energy = 0.0; return RD_OK; }
     HIPCK(h, hipMemsetAsync(h->d_etot, 0, sizeof(double), h->stream));
-    if (h->tiled)
-        launch_tiled<RD_TILE_GRAD>(h, h->d_xyz_io, h->d_etot);
+    if (h->kern == RD_KERNEL_TILED)
+        launch_tiled<RD_MODE_GRAD>(h, h->d_xyz_io, h->d_etot);
+    else if (h->kern == RD_KERNEL_WAVE)
+        hipLaunchKernelGGL(k_force_wave<RD_MODE_GRAD>, dim3(nblk(h->nwaves, RD_WAVE_WAVES)), dim3(64 * RD_WAVE_WAVES),
+                           0, h->stream, h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P,
+                           h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st, h->d_noise, h->n, h->d_xyz_io,
+                           h->d_etot);
     else
-        hipLaunchKernelGGL(k_gradients, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps,
-                           h->d_start, h->d_P, h->d_pot1, h->n, h->d_xyz_io, h->d_etot);
+        hipLaunchKernelGGL(k_gradients, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start, h->d_P,
+                           h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_xyz_io, h->d_etot);
     HIPCK(h, hipGetLastError());
     std::vector<float> tmp(3 * (size_t)h->n);
     double e = 0.0;
@@ -937,7 +1120,7 @@ extern "C" int rd_noise_fill(rd_system* h, uint64_t step0, int nsteps, float* ou
     float* d = nullptr;
     HIPCK(h, hipMalloc(&d, sizeof(float) * 3 * total));
     hipLaunchKernelGGL(k_noise_fill, dim3((unsigned)((total + RD_BLOCK - 1) / RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream,
-                       h->d_P, (unsigned long long)step0, nsteps, h->n, d);
+                       h->d_P, h->d_reps, (int)h->rep_dev.size(), (unsigned long long)step0, nsteps, h->n, d);
     hipError_t e1 = hipMemcpyAsync(out, d, sizeof(float) * 3 * total, hipMemcpyDeviceToHost, h->stream);
     hipError_t e2 = hipStreamSynchronize(h->stream);
     (void)hipFree(d);
@@ -968,31 +1151,32 @@ extern "C" int rd_set_noise_array(rd_system* h, const float* noise, int nsteps)
     st.err &= ~2u;
     HIPCK(h, hipMemcpyAsync(h->d_st, &st, sizeof(st), hipMemcpyHostToDevice, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
-    drop_graph(h);   // the captured kernel holds the noise pointer
+    drop_graph(h);   // the captured kernels hold the noise pointer
     return RD_OK;
 }
 
-// ---- RDF ----
+// ---- RDF (per replica) ----
 extern "C" int rd_rdf_configure(rd_system* h, double rmax, int nbins)
 {
     if (!h || !(rmax > 0) || nbins < 1) return RD_EINVAL;
-    if (h->nt == 0) return set_err(h, RD_ESTATE, "set particle types first");
-    for (int d = 0; d < 3; ++d)
-        if (rmax > 0.5 * (h->prm.lattice_bounds[2 * d + 1] - h->prm.lattice_bounds[2 * d]))
-            return set_err(h, RD_EINVAL, "RDF range exceeds half the box");
+    if (h->nt == 0 || h->reps.empty()) return set_err(h, RD_ESTATE, "set particle types and particles first");
+    for (const auto& A : h->reps)
+        for (int d = 0; d < 3; ++d)
+            if (rmax > 0.5 * (A.hi[d] - A.lo[d])) return set_err(h, RD_EINVAL, "RDF range exceeds half a box");
     h->rdf_rmax = rmax;
     h->rdf_bins = nbins;
-    size_t nh = (size_t)h->nt * h->nt * nbins;
+    h->rdf_nrep = (int)h->reps.size();
+    size_t nh = (size_t)h->rdf_nrep * h->nt * h->nt * nbins;
     if (h->d_rdf) (void)hipFree(h->d_rdf);
     h->d_rdf = nullptr;
     HIPCK(h, hipMalloc(&h->d_rdf, sizeof(unsigned long long) * nh));
     HIPCK(h, hipMemsetAsync(h->d_rdf, 0, sizeof(unsigned long long) * nh, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
     h->rdf_frames = 0;
-    h->rdf_npt_sum.assign(h->nt, 0.0);
-    h->rdf_pairs_sum.assign((size_t)h->nt * h->nt, 0.0);
-    h->rdf_vol = 1.0;
-    for (int d = 0; d < 3; ++d) h->rdf_vol *= h->prm.lattice_bounds[2 * d + 1] - h->prm.lattice_bounds[2 * d];
+    h->rdf_pairs_sum.assign((size_t)h->rdf_nrep * h->nt * h->nt, 0.0);
+    h->rdf_vol.assign(h->rdf_nrep, 1.0);
+    for (int r = 0; r < h->rdf_nrep; ++r)
+        for (int d = 0; d < 3; ++d) h->rdf_vol[r] *= h->reps[r].hi[d] - h->reps[r].lo[d];
     return RD_OK;
 }
 
@@ -1008,27 +1192,31 @@ extern "C" int rd_rdf_calculate(rd_system* h)
     if (!h->d_rdf) return set_err(h, RD_ESTATE, "rd_rdf_configure first");
     int rc = refresh(h);
     if (rc) return rc;
+    if ((int)h->rep_dev.size() != h->rdf_nrep) return set_err(h, RD_ESTATE, "replicas changed: reconfigure the RDF");
     int S = std::max(1, (int)std::ceil(h->rdf_rmax * 1.0001 / h->cell));
     size_t nh = (size_t)h->nt * h->nt * h->rdf_bins;
-    size_t lds = nh * sizeof(uint32_t);
+    size_t lds = nh * sizeof(uint32_t) * (RD_BLOCK / 64);
     if (h->n > 0) {
         if (lds <= 160 * 1024) {
             if (lds > 64 * 1024)
                 HIPCK(h, hipFuncSetAttribute((const void*)k_rdf<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(k_rdf<true>, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), lds, h->stream, h->d_ps,
-                               h->d_start, h->d_P, S, (float)h->rdf_rmax, h->rdf_bins, h->n, h->d_rdf);
-        } else
-            hipLaunchKernelGGL(k_rdf<false>, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps,
-                               h->d_start, h->d_P, S, (float)h->rdf_rmax, h->rdf_bins, h->n, h->d_rdf);
+            hipLaunchKernelGGL(k_rdf<true>, dim3(wave_blocks(h)), dim3(RD_BLOCK), lds, h->stream, h->d_ps, h->d_start,
+                               h->d_P, h->d_reps, h->d_waves, h->nwaves, S, (float)h->rdf_rmax, h->rdf_bins, h->d_rdf);
+        } else {
+            hipLaunchKernelGGL(k_rdf<false>, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
+                               h->d_P, h->d_reps, h->d_waves, h->nwaves, S, (float)h->rdf_rmax, h->rdf_bins, h->d_rdf);
+        }
         HIPCK(h, hipGetLastError());
     }
     // per-frame normalisation constants (particle numbers may change by reactions)
-    std::vector<long> npt(h->nt, 0);
-    for (int i = 0; i < h->n; ++i) npt[h->htype[i]]++;
-    for (int a = 0; a < h->nt; ++a) {
-        h->rdf_npt_sum[a] += (double)npt[a];
-        for (int b = 0; b < h->nt; ++b)
-            h->rdf_pairs_sum[(size_t)a * h->nt + b] += (double)npt[a] * (double)(npt[b] - (a == b ? 1 : 0));
+    const int nt = h->nt;
+    for (int r = 0; r < h->rdf_nrep; ++r) {
+        std::vector<long> npt(nt, 0);
+        const RdRep& R = h->rep_dev[r];
+        for (int i = 0; i < R.n; ++i) npt[h->htype[R.off + i]]++;
+        for (int a = 0; a < nt; ++a)
+            for (int b = 0; b < nt; ++b)
+                h->rdf_pairs_sum[((size_t)r * nt + a) * nt + b] += (double)npt[a] * (double)(npt[b] - (a == b ? 1 : 0));
     }
     h->rdf_frames++;
     HIPCK(h, hipStreamSynchronize(h->stream));
@@ -1039,33 +1227,34 @@ extern "C" int rd_rdf_copy(rd_system* h, uint64_t* counts, long* frames)
 {
     if (!h || !counts) return RD_EINVAL;
     if (!h->d_rdf) return set_err(h, RD_ESTATE, "rd_rdf_configure first");
-    size_t nh = (size_t)h->nt * h->nt * h->rdf_bins;
+    size_t nh = (size_t)h->rdf_nrep * h->nt * h->nt * h->rdf_bins;
     HIPCK(h, hipMemcpyAsync(counts, h->d_rdf, sizeof(uint64_t) * nh, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
     if (frames) *frames = h->rdf_frames;
     return RD_OK;
 }
 
-// g_ab(r_k) = C_ab(k) / (sum_frames N_a (N_b - [a==b]) / V * V_shell(k)), FP64.
+// g_ab(r_k) = C_ab(k) / (sum_frames N_a (N_b - [a==b]) / V * V_shell(k)), FP64, per replica.
 extern "C" int rd_rdf_normalize(rd_system* h, double* g)
 {
     if (!h || !g) return RD_EINVAL;
     if (!h->d_rdf) return set_err(h, RD_ESTATE, "rd_rdf_configure first");
     int nt = h->nt, nb = h->rdf_bins;
-    std::vector<uint64_t> c((size_t)nt * nt * nb);
+    std::vector<uint64_t> c((size_t)h->rdf_nrep * nt * nt * nb);
     int rc = rd_rdf_copy(h, c.data(), nullptr);
     if (rc) return rc;
     const double pi = 3.14159265358979323846;
     double dr = h->rdf_rmax / nb;
-    for (int a = 0; a < nt; ++a)
-        for (int b = 0; b < nt; ++b)
-            for (int k = 0; k < nb; ++k) {
-                double r1 = k * dr, r2 = (k + 1) * dr;
-                double vs = 4.0 / 3.0 * pi * (r2 * r2 * r2 - r1 * r1 * r1);
-                double den = h->rdf_pairs_sum[(size_t)a * nt + b] / h->rdf_vol * vs;
-                size_t idx = ((size_t)a * nt + b) * nb + k;
-                g[idx] = den > 0 ? (double)c[idx] / den : 0.0;
-            }
+    for (int r = 0; r < h->rdf_nrep; ++r)
+        for (int a = 0; a < nt; ++a)
+            for (int b = 0; b < nt; ++b)
+                for (int k = 0; k < nb; ++k) {
+                    double r1 = k * dr, r2 = (k + 1) * dr;
+                    double vs = 4.0 / 3.0 * pi * (r2 * r2 * r2 - r1 * r1 * r1);
+                    double den = h->rdf_pairs_sum[((size_t)r * nt + a) * nt + b] / h->rdf_vol[r] * vs;
+                    size_t idx = (((size_t)r * nt + a) * nt + b) * nb + k;
+                    g[idx] = den > 0 ? (double)c[idx] / den : 0.0;
+                }
     return RD_OK;
 }
 
@@ -1091,8 +1280,9 @@ static int pairs_common(rd_system* h, bool reactions, int64_t* out, long cap, lo
             h->pairs_cap = c0;
         }
         HIPCK(h, hipMemsetAsync(h->d_pcount, 0, sizeof(unsigned long long), h->stream));
-        hipLaunchKernelGGL(k_pairs, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
-                           h->d_P, dtable, S, h->n, h->d_pairs, (unsigned long long)h->pairs_cap, h->d_pcount);
+        hipLaunchKernelGGL(k_pairs, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start, h->d_P,
+                           h->d_reps, h->d_waves, h->nwaves, dtable, S, h->d_pairs, (unsigned long long)h->pairs_cap,
+                           h->d_pcount);
         HIPCK(h, hipGetLastError());
         unsigned long long total = 0;
         HIPCK(h, hipMemcpyAsync(&total, h->d_pcount, sizeof(total), hipMemcpyDeviceToHost, h->stream));
@@ -1127,11 +1317,12 @@ extern "C" int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long
     return pairs_common(h, true, out, cap, count);
 }
 
-// ---- reactions: lib/ReaDDyGPU.cpp:303-463 ----
+// ---- reactions: lib/ReaDDyGPU.cpp:303-463 (single system) ----
 extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
 {
     if (!h || !cr) return RD_EINVAL;
     if (!h->initialized) return set_err(h, RD_ESTATE, "not initialized");
+    if (h->reps.size() != 1) return set_err(h, RD_ESTATE, "reactions need a single-system handle");
     int nr = (int)cr[0];
     if (nr <= 0) return RD_OK;
     int rc = rd_copy_pos_from_device(h, nullptr, nullptr);
@@ -1171,10 +1362,13 @@ extern "C" int rd_get_info(rd_system* h, rd_info* info)
     memset(info, 0, sizeof(*info));
     info->n = h->n;
     info->n_types = h->nt;
-    for (int d = 0; d < 3; ++d) info->lattice[d] = h->P.L[d];
+    if (!h->rep_dev.empty())
+        for (int d = 0; d < 3; ++d) info->lattice[d] = h->rep_dev[0].L[d];
     info->cell_size = h->cell;
     info->stencil = h->P.S;
     info->step = h->step;
     info->rdf_frames = h->rdf_frames;
+    info->n_replicas = (int)h->reps.size();
+    info->n_cells = h->ncell;
     return RD_OK;
 }

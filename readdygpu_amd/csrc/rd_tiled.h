@@ -10,7 +10,7 @@
 // Replaces warpOrderTwo (CudaSimulation.cu:1402-1782), which walks per-cell
 // linked lists with 32-lane blocks and global float atomics.
 #pragma once
-#include "rd_kernels.h"
+#include "rd_wave.h"
 
 struct RdTileGeom {
     int TY;      // rows per tile
@@ -43,7 +43,6 @@ __host__ __device__ inline size_t rd_tile_lds_bytes(const RdTileGeom& G)
     return b;
 }
 
-enum { RD_TILE_UPDATE = 0, RD_TILE_GRAD = 1 };
 
 template <int BLOCK, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_force_tiled(const float4* __restrict__ ps, float4* __restrict__ pn,
@@ -51,6 +50,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_tiled(const float4* __restrict_
                                                        uint32_t* __restrict__ count,
                                                        const uint32_t* __restrict__ start,
                                                        const RdDev* __restrict__ P,
+                                                       const RdRep* __restrict__ reps,
                                                        const RdPot1Dev* __restrict__ pot1,
                                                        RdStepState* __restrict__ st,
                                                        const float* __restrict__ noise, int n, RdTileGeom G,
@@ -69,7 +69,8 @@ __global__ __launch_bounds__(BLOCK) void k_force_tiled(const float4* __restrict_
 
     load_tab(T, P);   // ends with a barrier
 
-    const int Lx = P->L[0], Ly = P->L[1], Lz = P->L[2];
+    const RdRep RG = reps[0];   // the tiled kernel runs single-replica systems
+    const int Lx = RG.L[0], Ly = RG.L[1], Lz = RG.L[2];
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int seg = tile % G.nseg;
     const int yt = (tile / G.nseg) % G.nyt;
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_tiled(const float4* __restrict_
         int yy = y0 - 1 + yi, zz = z - 1 + dzi;
         yy = (yy % Ly + Ly) % Ly;
         zz = (zz % Lz + Lz) % Lz;
-        int row = (zz * Ly + yy) * Lx;
+        int row = RG.coff + (zz * Ly + yy) * Lx;
         int g0 = (int)start[row + xs];
         int nA, g1 = 0, ntot;
         if (xs + W <= Lx) {
@@ -145,8 +146,6 @@ __global__ __launch_bounds__(BLOCK) void k_force_tiled(const float4* __restrict_
         hcount += staged ? lsx[k * (W + 1) + ci0 + twe] - lsx[k * (W + 1) + ci0]
                          : 0;
     }
-    const float Lbx = P->Lb[0], Lby = P->Lb[1], Lbz = P->Lb[2];
-    const float hxb = P->half[0], hyb = P->half[1], hzb = P->half[2];
     double eblk = 0.0;
 
     if (staged) {
@@ -167,7 +166,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_tiled(const float4* __restrict_
             const float4 p = buf[l];
             const int ti = w_type(p.w);
             const int rowbase = ti * P->nt;
-            const int cx = cell_coord(p.x, P->lo[0], P->cell, Lx);
+            const int cx = cell_coord(p.x, RG.lo[0], RG.cell, Lx);
             int ci = cx - xs;
             if (ci < 0) ci += Lx;
             long long ax = 0, ay = 0, az = 0;
@@ -176,7 +175,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_tiled(const float4* __restrict_
                 for (int j = a; j < b; ++j) {
                     float4 q = buf[j];
                     if (__float_as_uint(q.w) != __float_as_uint(p.w))
-                        pair_accum<MODE == RD_TILE_GRAD>(T, rowbase, p, q, Lbx, Lby, Lbz, hxb, hyb, hzb, ax, ay, az, e);
+                        pair_accum<MODE == RD_MODE_GRAD>(T, rowbase, p, q, RG, ax, ay, az, e);
                 }
             };
             if (P->n_pot2 > 0) {
@@ -203,36 +202,36 @@ __global__ __launch_bounds__(BLOCK) void k_force_tiled(const float4* __restrict_
             for (int kk = 0; kk < np1; ++kk) {
                 const RdPot1Dev& q = pot1[kk];
                 if (!((q.mask >> ti) & 1u)) continue;
-                ext_grad<MODE == RD_TILE_GRAD>(q, prad, p.x, p.y, p.z, gx, gy, gz, &e);
+                ext_grad<MODE == RD_MODE_GRAD>(q, prad, p.x, p.y, p.z, gx, gy, gz, &e);
             }
-            if (MODE == RD_TILE_GRAD) {
-                const uint32_t idx = w_index(p.w);
+            if (MODE == RD_MODE_GRAD) {
+                const uint32_t idx = (uint32_t)RG.off + w_index(p.w);
                 grad[3 * idx] = gx; grad[3 * idx + 1] = gy; grad[3 * idx + 2] = gz;
                 eblk += (double)e;
             } else {
-                em_update_store(P, T, st, noise, n, s, p, make_float3(gx, gy, gz), pn, key, rank, count);
+                em_update_store(P, RG, T, st, noise, n, s, p, make_float3(gx, gy, gz), pn, key, rank, count);
             }
         }
     } else {
         // the tile overflowed its LDS budget: gather from global memory
         for (int yi = 1; yi <= hy; ++yi) {
-            int row = ((z * Ly) + (y0 + yi - 1)) * Lx;
+            int row = RG.coff + ((z * Ly) + (y0 + yi - 1)) * Lx;
             uint32_t s0 = start[row + x0], s1 = start[row + x0 + twe];
             for (uint32_t s = s0 + threadIdx.x; s < s1; s += BLOCK) {
                 float4 p = ps[s];
                 float e = 0.0f;
-                float3 g = gather_grad<MODE == RD_TILE_GRAD>(P, T, pot1, ps, start, s, p, &e);
-                if (MODE == RD_TILE_GRAD) {
-                    const uint32_t idx = w_index(p.w);
+                float3 g = gather_grad<MODE == RD_MODE_GRAD>(P, RG, T, pot1, ps, start, s, p, &e);
+                if (MODE == RD_MODE_GRAD) {
+                    const uint32_t idx = (uint32_t)RG.off + w_index(p.w);
                     grad[3 * idx] = g.x; grad[3 * idx + 1] = g.y; grad[3 * idx + 2] = g.z;
                     eblk += (double)e;
                 } else {
-                    em_update_store(P, T, st, noise, n, s, p, g, pn, key, rank, count);
+                    em_update_store(P, RG, T, st, noise, n, s, p, g, pn, key, rank, count);
                 }
             }
         }
     }
-    if (MODE == RD_TILE_GRAD) {
+    if (MODE == RD_MODE_GRAD) {
         __shared__ double red[BLOCK / 64];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) eblk += __shfl_down(eblk, o, 64);

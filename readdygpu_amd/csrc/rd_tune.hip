@@ -7,18 +7,18 @@
 #include <cstring>
 #include <vector>
 
-int rd_host_refresh(rd_system* h);
-int rd_host_set_step(rd_system* h, uint64_t step);
-
 static int time_steps(rd_system* h, int nsteps, double* ms_per_step)
 {
-    int rc = rd_simulate(h, 2);          // builds the graph, warms caches
+    int rc = rd_simulate(h, 2);          // warms caches
+    if (rc) return rc;
+    hipGraphExec_t ex = nullptr;
+    rc = rd_host_step_graph(h, 1, &ex);
     if (rc) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float ms = 0.0f;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return RD_EHIP;
     (void)hipEventRecord(e0, h->stream);
-    for (int i = 0; i < nsteps; ++i) (void)hipGraphLaunch(h->graph_exec, h->stream);
+    for (int i = 0; i < nsteps; ++i) (void)hipGraphLaunch(ex, h->stream);
     (void)hipEventRecord(e1, h->stream);
     hipError_t e = hipEventSynchronize(e1);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -51,8 +51,8 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     else
         for (double f : { 1.01, 1.2, 1.45, 1.75, 2.1 }) cells.push_back(f * base);
     struct Cfg { int kernel, block; double fill; };
-    const Cfg cfgs[] = { { RD_KERNEL_GATHER, 256, 0.0 }, { RD_KERNEL_TILED, 128, 0.6 }, { RD_KERNEL_TILED, 256, 0.6 },
-                         { RD_KERNEL_TILED, 256, 1.0 }, { RD_KERNEL_TILED, 512, 0.8 } };
+    const Cfg cfgs[] = { { RD_KERNEL_WAVE, 256, 0.0 }, { RD_KERNEL_GATHER, 256, 0.0 }, { RD_KERNEL_TILED, 128, 0.6 },
+                         { RD_KERNEL_TILED, 256, 0.6 } };
     double best = 1e30, best_cell = cells[0];
     Cfg best_cfg = cfgs[0];
     for (double c : cells) {
@@ -61,7 +61,7 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
             rd_launch_config lc{};
             lc.kernel = k.kernel; lc.block = k.block; lc.fill = k.fill;
             rc = rd_set_launch_config(h, &lc);
-            if (rc == RD_OK) rc = rd_set_particles(h, (int)st.size(), sx.data(), st.data());
+            if (rc == RD_OK) rc = rd_host_set_state(h, sx, st);
             if (rc == RD_OK) rc = rd_host_refresh(h);
             if (rc == RD_ERANGE) continue;          // too many cells for this size
             if (rc) return rc;
@@ -77,7 +77,7 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     rd_launch_config lc = user_lc;
     lc.kernel = best_cfg.kernel; lc.block = best_cfg.block; lc.fill = best_cfg.fill;
     rc = rd_set_launch_config(h, &lc);
-    if (rc == RD_OK) rc = rd_set_particles(h, (int)st.size(), sx.data(), st.data());
+    if (rc == RD_OK) rc = rd_host_set_state(h, sx, st);
     if (rc == RD_OK) rc = rd_host_refresh(h);
     if (rc) return rc;
     rc = rd_host_set_step(h, s0);

@@ -26,10 +26,17 @@ def _ptr(a: Optional[np.ndarray]):
 
 
 class Simulation:
-    """One periodic BD system on one GPU (Simulation + CudaSimulation)."""
+    """Periodic BD on one GPU (Simulation + CudaSimulation).
+
+    A plain system uses the box of ``spec``.  With ``replicas=(bounds, counts)``
+    or ``(bounds, counts, streams)`` the handle holds independent boxes
+    (bounds [R,6] as lo,hi per axis, counts [R] particles each, stored
+    consecutively in ``xyz``/``types``) that advance together; replica r uses
+    noise stream ``streams[r]`` (default r).
+    """
 
     def __init__(self, spec: SystemSpec, xyz: np.ndarray, types: np.ndarray, device: int = 0,
-                 rdf: Optional[tuple] = None, initialize: bool = True):
+                 rdf: Optional[tuple] = None, initialize: bool = True, replicas: Optional[tuple] = None):
         self.lib = _lib.load()
         self.spec = spec
         prm = _lib.rd_params()
@@ -75,7 +82,10 @@ class Simulation:
         if spec.reaction_radii is not None:
             R = np.ascontiguousarray(spec.reaction_radii, dtype=np.float64)
             self._check(self.lib.rd_set_reaction_radii(h, _ptr(R)))
-        self.set_particles(xyz, types)
+        if replicas is None:
+            self.set_particles(xyz, types)
+        else:
+            self.set_replicas(*replicas[:2], xyz, types, streams=replicas[2] if len(replicas) > 2 else None)
         if rdf is not None:
             self.configure_rdf(*rdf)
         if initialize:
@@ -112,6 +122,22 @@ class Simulation:
         if len(xyz) != len(types):
             raise ValueError("xyz and types differ in length")
         self._check(self.lib.rd_set_particles(self.h, len(types), _ptr(xyz), _ptr(types)))
+
+    def set_replicas(self, bounds: np.ndarray, counts: np.ndarray, xyz: np.ndarray, types: np.ndarray,
+                     streams: Optional[np.ndarray] = None):
+        """Independent boxes in one handle (no counterpart in the reference)."""
+        b = np.ascontiguousarray(bounds, dtype=np.float64).reshape(-1, 6)
+        c = np.ascontiguousarray(counts, dtype=np.int32).reshape(-1)
+        if len(b) != len(c):
+            raise ValueError("bounds and counts differ in length")
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        types = np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
+        if len(xyz) != len(types) or len(types) != int(c.sum()):
+            raise ValueError("particle arrays do not match the replica counts")
+        st = None if streams is None else np.ascontiguousarray(streams, dtype=np.uint32).reshape(-1)
+        if st is not None and len(st) != len(c):
+            raise ValueError("streams and counts differ in length")
+        self._check(self.lib.rd_set_replicas(self.h, len(c), _ptr(b), _ptr(st), _ptr(c), _ptr(xyz), _ptr(types)))
 
     def initialize(self):
         """CudaSimulation::initialize (CudaSimulation.cu:138-455)."""
@@ -162,7 +188,9 @@ class Simulation:
 
     def configure_rdf(self, r_max: float, n_bins: int):
         self._check(self.lib.rd_rdf_configure(self.h, float(r_max), int(n_bins)))
-        self._rdf_shape = (self.spec.n_types, self.spec.n_types, int(n_bins))
+        nt = self.spec.n_types
+        nrep = self.info().n_replicas
+        self._rdf_shape = (nt, nt, int(n_bins)) if nrep == 1 else (nrep, nt, nt, int(n_bins))
 
     def call_rdf_calculation(self):
         """callRDFCalculation (CudaSimulation.cu:505-520)."""

@@ -66,3 +66,30 @@ def plain_1m(n: int = 1_000_000, seed: int = 2, cell_size: float = 0.0, box: flo
 
 
 PLAIN_1M = dict(steps=500, rdf_range=20.0, rdf_bins=200)
+
+
+def batched_replicas(n_replicas: int = 256, seed: int = 3, nmin: int = 500, nmax: int = 50_000,
+                     phi: float = 0.05, cell_size: float = 0.0):
+    """256 independent periodic boxes, particle counts log-uniform in
+    [500, 50000], 2 species as plain_10k (radii 1.5/2.0 nm, D 1.0/0.5), harmonic
+    repulsion k=10, dt=0.1 ns; each box edge makes the volume fraction 0.05.
+    Returns (spec, boxes [R,6], counts [R], xyz [sum N,3], types [sum N]);
+    replica r uses noise stream r."""
+    spec = SystemSpec(box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, 1.0, 1.0), dt=0.1, seed=seed, cell_size=cell_size)
+    spec.types = [ParticleType("A", D=1.0, radius=1.5), ParticleType("B", D=0.5, radius=2.0)]
+    spec.pot2 = [OrderTwoPotential(HARMONIC, REPULSIVE, k=10.0)]
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_replicas))).astype(np.int64)
+    boxes = np.zeros((n_replicas, 6))
+    xyz, types = [], []
+    vol_a, vol_b = 4.0 / 3.0 * np.pi * 1.5 ** 3, 4.0 / 3.0 * np.pi * 2.0 ** 3
+    for r, n in enumerate(counts):
+        na = n // 2
+        L = float(np.cbrt((na * vol_a + (n - na) * vol_b) / phi))
+        boxes[r] = (0.0, L, 0.0, L, 0.0, L)
+        xyz.append(uniform_positions(rng, int(n), (0.0, 0.0, 0.0), (L, L, L)))
+        types.append(np.repeat(np.arange(2, dtype=np.int32), [na, n - na]))
+    return spec, boxes, counts, np.concatenate(xyz), np.concatenate(types)
+
+
+BATCHED = dict(steps=1000, rdf_range=20.0, rdf_bins=200)

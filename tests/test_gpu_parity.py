@@ -152,8 +152,9 @@ def test_launch_configs_give_identical_trajectories(which, c1):
         spec, xyz, types = workloads.plain_1m(n=200_000, box=585.0)
     ref = None
     rc = float(spec.pair_range().max())
-    shapes = [dict(kernel=1), dict(kernel=2, block=64, fill=0.5), dict(kernel=2, block=128, rows_per_tile=1),
-              dict(kernel=2, block=256, fill=1.0), dict(kernel=2, block=512, fill=0.3)]
+    shapes = [dict(kernel=3), dict(kernel=1), dict(kernel=2, block=64, fill=0.5),
+              dict(kernel=2, block=128, rows_per_tile=1), dict(kernel=2, block=256, fill=1.0),
+              dict(kernel=2, block=512, fill=0.3)]
     for cell in (0.0, 1.3 * rc):
         for shape in shapes:
             spec.cell_size = cell
