@@ -176,6 +176,50 @@ int rd_apply_reactions(rd_system* h, const float* creactions);
 
 int rd_get_info(rd_system* h, rd_info* info);
 
+/* ---- composite external potential and its learned surrogate ----
+ * No counterpart in the reference.  The composite is a geometry of order-one
+ * pieces: attractive disks and cylinders (the forms of orderOne,
+ * CudaSimulation.cu:1049-1148, with any normal).  A particle whose type is in
+ * type_mask feels the piece nearest to it in energy, U = min_i U_i, with the
+ * gradient of that piece (first index on ties).  It is evaluated exactly by a
+ * pass over all pieces, or by a BF16 MLP 3-64-64-64-3 (SiLU) trained on the
+ * GPU on gradients of the exact composite.  Both add to the particle's
+ * gradient before the Euler-Maruyama update. */
+enum { RD_EXT_EXACT = 0, RD_EXT_SURROGATE = 1 };
+int rd_set_composite(rd_system* h, int n_pieces, const rd_pot1* pieces, uint32_t type_mask);
+int rd_set_external_mode(rd_system* h, int mode);
+/* exact composite at m host positions: grad[3m], energy[m] (either may be NULL) */
+int rd_composite_eval(rd_system* h, long m, const float* xyz, float* grad, float* energy);
+
+/* Surrogate parameters: FP32 master vector of 8771 values,
+ * W1[64][3] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[3][64] b4[3].
+ * Input xn = (x - lo) * (2 / (hi - lo)) - 1 per axis (FP32); output times
+ * gscale = dU/dx. */
+typedef struct {
+    int batch;             /* samples per optimiser step (multiple of 64 is best) */
+    int steps;             /* optimiser steps */
+    float lr, lr_min;      /* cosine schedule from lr to lr_min */
+    float beta1, beta2, eps;
+    uint64_t seed;         /* minibatch sampling */
+} rd_train_params;
+int rd_surrogate_init(rd_system* h, uint64_t seed);
+int rd_surrogate_set_weights(rd_system* h, const float* w);
+int rd_surrogate_get_weights(rd_system* h, float* w);
+int rd_surrogate_set_norm(rd_system* h, const double* lo3, const double* hi3, double gscale);
+int rd_surrogate_get_norm(rd_system* h, double* lo3, double* hi3, double* gscale);
+/* m training samples: uniform positions in the domain, exact gradients; sets gscale */
+int rd_surrogate_make_data(rd_system* h, long m, uint64_t seed, double* gscale);
+/* Adam over the training samples; loss of every step into loss_curve[steps] (may be NULL) */
+int rd_surrogate_train(rd_system* h, const rd_train_params* params, float* loss_curve);
+/* loss (1/m) sum |y - t|^2 and its gradient for one batch of normalised
+ * inputs xn[3m] and targets t[3m], without an update */
+int rd_surrogate_loss_grad(rd_system* h, long m, const float* xn, const float* t, double* loss, float* grad);
+/* raw network outputs y[3m] at normalised inputs xn[3m] */
+int rd_surrogate_predict(rd_system* h, long m, const float* xn, float* y);
+/* relative RMSE of the surrogate gradient against the exact composite on m
+ * fresh uniform samples */
+int rd_surrogate_validate(rd_system* h, long m, uint64_t seed, double* rel_rmse);
+
 /* ---- input files: the reference's XML schema (Xerces-free reader) ----
  * rd_model is the parsed Simulation of Simulation::readXML / readGlobalXML /
  * parse (Simulation.cpp:206-338): param_global (GlobalParametersHandler.cpp:

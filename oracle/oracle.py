@@ -70,6 +70,11 @@ def lib():
             "orc_pairs_cells": (C.c_long, [V, C.c_int, V, V, V, V, C.c_long]),
             "orc_rdf_counts": (None, [V, C.c_int, V, V, C.c_float, C.c_int, V, C.c_int]),
             "orc_rdf_normalize": (None, [C.c_int, C.c_int, C.c_double, C.c_double, V, C.c_long, V, V]),
+            "orc_piece_eval": (C.c_double, [V, V, V]),
+            "orc_composite_eval": (None, [V, C.c_int, C.c_long, V, V, V, V, V, C.c_int]),
+            "orc_bf16": (C.c_float, [C.c_float]),
+            "orc_mlp_forward": (None, [V, C.c_long, V, C.c_int, V]),
+            "orc_mlp_loss_grad": (C.c_double, [V, C.c_long, V, V, C.c_int, V]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -217,3 +222,75 @@ def rdf_normalize(nt, nbins, rmax, volume, npt, frames, counts):
     g = np.zeros(counts.shape, dtype=np.float64)
     lib().orc_rdf_normalize(nt, nbins, float(rmax), float(volume), _p(npt), int(frames), _p(counts), _p(g))
     return g
+
+
+# ---------------------------------------------------------------------------
+# composite external potential and MLP surrogate (rd_oracle_mlp.c)
+# ---------------------------------------------------------------------------
+MLP_NPARAM = 8771
+
+
+def pieces_array(pieces):
+    """orc_pot1 array from readdygpu_amd.model.OrderOnePotential pieces; the
+    parameters are taken at FP32, as the GPU stores them."""
+    arr = (orc_pot1 * len(pieces))()
+    f = np.float32
+    for i, p in enumerate(pieces):
+        q = arr[i]
+        q.kind, q.subtype, q.k = p.kind, p.subtype, float(f(p.k))
+        for d in range(3):
+            q.origin[d] = float(f(p.origin[d]))
+            q.normal[d] = float(f(p.normal[d]))
+            q.extension[d] = float(f(p.extension[d]))
+        q.height, q.radius = float(f(p.height)), float(f(p.radius))
+        q.type_mask = 0xFFFFFFFFFFFFFFFF
+    return arr
+
+
+def composite(pieces, xyz, nthreads=0):
+    """Nearest-piece composite at FP64: (grad [m,3], energy [m], which [m], gap [m])."""
+    arr = pieces_array(pieces)
+    x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    m = len(x)
+    g = np.zeros((m, 3))
+    e = np.zeros(m)
+    w = np.zeros(m, dtype=np.int32)
+    gap = np.zeros(m)
+    lib().orc_composite_eval(C.cast(arr, C.c_void_p), len(pieces), m, _p(x), _p(g), _p(e), _p(w), _p(gap),
+                             int(nthreads))
+    return g, e, w, gap
+
+
+def piece_eval(piece, x):
+    arr = pieces_array([piece])
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    g = np.zeros(3)
+    e = lib().orc_piece_eval(C.cast(arr, C.c_void_p), _p(xx), _p(g))
+    return e, g
+
+
+def bf16(v):
+    """Round FP32 values to BF16 (nearest even), returned as float32."""
+    a = np.ascontiguousarray(v, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    out = r.view(np.float32)
+    special = (a.view(np.uint32) & 0x7F800000) == 0x7F800000
+    return np.where(special, a, out)
+
+
+def mlp_forward(w, xn, bf16_round=False):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.ascontiguousarray(xn, dtype=np.float32).reshape(-1, 3)
+    y = np.zeros((len(x), 3))
+    lib().orc_mlp_forward(_p(w), len(x), _p(x), int(bf16_round), _p(y))
+    return y
+
+
+def mlp_loss_grad(w, xn, t, bf16_round=False):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.ascontiguousarray(xn, dtype=np.float32).reshape(-1, 3)
+    tt = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 3)
+    g = np.zeros(MLP_NPARAM)
+    loss = lib().orc_mlp_loss_grad(_p(w), len(x), _p(x), _p(tt), int(bf16_round), _p(g))
+    return loss, g

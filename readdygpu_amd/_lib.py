@@ -99,6 +99,19 @@ class rd_info(C.Structure):
     ]
 
 
+class rd_train_params(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int),
+        ("steps", C.c_int),
+        ("lr", C.c_float),
+        ("lr_min", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("eps", C.c_float),
+        ("seed", C.c_uint64),
+    ]
+
+
 P = C.c_void_p
 _SIGS = {
     "rd_create": (C.c_int, [C.POINTER(rd_params), C.POINTER(P)]),
@@ -137,6 +150,19 @@ _SIGS = {
     "rd_model_pot1": (C.c_int, [P, C.c_int, C.POINTER(rd_pot1)]),
     "rd_model_pot2": (C.c_int, [P, C.c_int, C.POINTER(rd_pot2), P]),
     "rd_create_from_model": (C.c_int, [P, C.c_double, C.c_uint64, C.POINTER(P)]),
+    "rd_set_composite": (C.c_int, [P, C.c_int, P, C.c_uint32]),
+    "rd_set_external_mode": (C.c_int, [P, C.c_int]),
+    "rd_composite_eval": (C.c_int, [P, C.c_long, P, P, P]),
+    "rd_surrogate_init": (C.c_int, [P, C.c_uint64]),
+    "rd_surrogate_set_weights": (C.c_int, [P, P]),
+    "rd_surrogate_get_weights": (C.c_int, [P, P]),
+    "rd_surrogate_set_norm": (C.c_int, [P, P, P, C.c_double]),
+    "rd_surrogate_get_norm": (C.c_int, [P, P, P, C.POINTER(C.c_double)]),
+    "rd_surrogate_make_data": (C.c_int, [P, C.c_long, C.c_uint64, C.POINTER(C.c_double)]),
+    "rd_surrogate_train": (C.c_int, [P, C.POINTER(rd_train_params), P]),
+    "rd_surrogate_loss_grad": (C.c_int, [P, C.c_long, P, P, C.POINTER(C.c_double), P]),
+    "rd_surrogate_predict": (C.c_int, [P, C.c_long, P, P]),
+    "rd_surrogate_validate": (C.c_int, [P, C.c_long, C.c_uint64, C.POINTER(C.c_double)]),
     "rd_bench_steps": (C.c_int, [P, C.c_int, P]),
     "rd_bench_copy": (C.c_int, [P, C.c_long, C.c_int, C.POINTER(C.c_double)]),
 }

@@ -63,6 +63,7 @@ struct RdDev {
     float rr2[RD_MAXT * RD_MAXT];   // squared reaction radius (0: none)
     uint32_t p2mask[RD_MAXT * RD_MAXT];
     RdPot2Dev pot2[RD_MAXP2];
+    const float4* gext;   // external gradient per slot (x,y,z, energy) from a separate pass, or null
 };
 
 // packed particle word: low 6 bits type, high 26 bits particle index

@@ -10,6 +10,7 @@
 
 #include "../../include/rdgpu.h"
 #include "rd_device.h"
+#include "rd_mlp.h"
 #include "rd_tiled.h"
 
 struct RdStepState;
@@ -108,6 +109,31 @@ struct rd_system {
     int tile_block = 256;
     RdTileGeom tg{};
     size_t tile_lds = 0;
+
+    // composite external potential (nearest piece) and its MLP surrogate
+    std::vector<rd_pot1> comp;
+    uint32_t comp_mask = 0;
+    int npieces = 0;
+    void* d_pieces = nullptr;
+    int ext_mode = RD_EXT_EXACT;
+    float4* d_gext = nullptr;        // external gradient per sorted slot
+    int gext_cap = 0;
+    float *d_master = nullptr, *d_mom = nullptr, *d_vel = nullptr;
+    void* d_pack = nullptr;          // RdMlpPack
+    void* d_ctl = nullptr;           // training control block
+    float* d_partial = nullptr;
+    size_t partial_rows = 0;
+    float4 *d_xn = nullptr, *d_tg = nullptr;
+    long nsamples = 0;
+    float* d_gscale = nullptr;
+    double* d_sbsum = nullptr;       // per-block sums of the training-data scale
+    float* d_loss_hist = nullptr;
+    size_t loss_hist_cap = 0;
+    RdMlpNorm mlp_norm{};
+    double norm_hi[3] = { 0, 0, 0 };
+    bool norm_set = false;
+    bool sur_ready = false;
+    bool ext_active() const { return npieces > 0 || (ext_mode == RD_EXT_SURROGATE && sur_ready); }
 };
 
 double rd_host_r0(const rd_system* h, int i, int j);
@@ -117,3 +143,6 @@ int rd_host_set_state(rd_system* h, const std::vector<float>& xyz, const std::ve
 int rd_host_step_graph(rd_system* h, int k, hipGraphExec_t* exec);
 void rd_host_launch_force(rd_system* h);
 void rd_host_launch_sort(rd_system* h, int advance);
+void rd_host_launch_ext(rd_system* h);
+int rd_host_ext_capacity(rd_system* h);
+void rd_host_free_surrogate(rd_system* h);

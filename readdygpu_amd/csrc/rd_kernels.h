@@ -158,15 +158,22 @@ __device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p
     }
 }
 
-// external potentials, added to the fixed-point pair sum
+// external potentials, added to the fixed-point pair sum; a gradient computed
+// by a separate external pass (composite geometry, surrogate) is added first
 template <bool ENERGY>
 __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const RdTab& T,
-                                              const RdPot1Dev* __restrict__ pot1, float4 p, long long ax,
+                                              const RdPot1Dev* __restrict__ pot1, uint32_t s, float4 p, long long ax,
                                               long long ay, long long az, float* e)
 {
     float gx = (float)((double)ax * RD_FIX_INV);
     float gy = (float)((double)ay * RD_FIX_INV);
     float gz = (float)((double)az * RD_FIX_INV);
+    const float4* __restrict__ gext = P->gext;
+    if (gext) {
+        const float4 q = gext[s];
+        gx = gx + q.x; gy = gy + q.y; gz = gz + q.z;
+        if (ENERGY) *e += q.w;
+    }
     const int ti = w_type(p.w);
     const int np1 = P->n_pot1;
     const float prad = T.prad[ti];
@@ -232,7 +239,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
             for_stencil(R, start, cx, cy, cz, P->S, [&](uint32_t j) { body(j, ps[j]); });
         }
     }
-    float3 g = finish_grad<ENERGY>(P, T, pot1, p, ax, ay, az, &e);
+    float3 g = finish_grad<ENERGY>(P, T, pot1, s, p, ax, ay, az, &e);
     if (ENERGY) *energy = e;
     return g;
 }

@@ -442,6 +442,7 @@ extern "C" void rd_destroy(rd_system* h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graph(h);
     free_dev(h);
+    rd_host_free_surrogate(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -782,7 +783,7 @@ static void choose_tiles(rd_system* h)
 {
     h->tiled = false;
     h->tile_block = h->lc.block > 0 ? h->lc.block : 256;
-    if (h->lc.kernel != RD_KERNEL_TILED || h->rep_dev.size() != 1) return;
+    if (h->lc.kernel != RD_KERNEL_TILED || h->rep_dev.size() != 1 || h->ext_active()) return;
     const RdRep& R0 = h->rep_dev[0];
     const RdDev& P = h->P;
     const int Lx = R0.L[0], Ly = R0.L[1], Lz = R0.L[2];
@@ -862,6 +863,7 @@ void rd_host_launch_force(rd_system* h)
 
 static void launch_step(rd_system* h)
 {
+    if (h->ext_active()) rd_host_launch_ext(h);
     rd_host_launch_force(h);
     launch_sort(h, 1);
 }
@@ -873,6 +875,9 @@ static int upload(rd_system* h)
     if (rc) return rc;
     rc = ensure_capacity(h);
     if (rc) return rc;
+    rc = rd_host_ext_capacity(h);
+    if (rc) return rc;
+    h->P.gext = h->ext_active() ? h->d_gext : nullptr;
     choose_tiles(h);
     h->kern = h->tiled ? RD_KERNEL_TILED : (h->lc.kernel == RD_KERNEL_GATHER ? RD_KERNEL_GATHER : RD_KERNEL_WAVE);
     HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
@@ -1089,6 +1094,7 @@ extern "C" int rd_gradients(rd_system* h, float* grad, double* energy)
     if (rc) return rc;
     if (h->n == 0) { if (energy) *energy = 0.0; return RD_OK; }
     HIPCK(h, hipMemsetAsync(h->d_etot, 0, sizeof(double), h->stream));
+    if (h->ext_active()) rd_host_launch_ext(h);
     if (h->kern == RD_KERNEL_TILED)
         launch_tiled<RD_MODE_GRAD>(h, h->d_xyz_io, h->d_etot);
     else if (h->kern == RD_KERNEL_WAVE)

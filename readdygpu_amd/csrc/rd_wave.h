@@ -164,7 +164,7 @@ __global__ __launch_bounds__(64 * RD_WAVE_WAVES) void k_force_wave(
 
     double eblk = 0.0;
     if (valid) {
-        float3 g = finish_grad<MODE == RD_MODE_GRAD>(P, T, pot1, p, ax, ay, az, &e);
+        float3 g = finish_grad<MODE == RD_MODE_GRAD>(P, T, pot1, (uint32_t)s, p, ax, ay, az, &e);
         if (MODE == RD_MODE_GRAD) {
             const uint32_t gi = (uint32_t)R.off + w_index(p.w);
             grad[3 * (size_t)gi] = g.x; grad[3 * (size_t)gi + 1] = g.y; grad[3 * (size_t)gi + 2] = g.z;

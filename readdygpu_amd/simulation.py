@@ -244,6 +244,93 @@ class Simulation:
         self._check(self.lib.rd_autotune(self.h, int(trial_steps), C.byref(ms)))
         return ms.value
 
+    # -- composite external potential and surrogate (no reference counterpart) --
+    def set_composite(self, pieces, types=None):
+        """Nearest-piece composite of attractive DISK / CYLINDER pieces
+        (model.OrderOnePotential); applies to `types` (default: all)."""
+        arr = (_lib.rd_pot1 * max(len(pieces), 1))()
+        for i, p in enumerate(pieces):
+            q = arr[i]
+            q.type, q.subtype, q.force_const = p.kind, p.subtype, p.k
+            for d in range(3):
+                q.origin[d] = p.origin[d]
+                q.normal[d] = p.normal[d]
+                q.extension[d] = p.extension[d]
+            q.height, q.radius = p.height, p.radius
+            q.type_mask = 0xFFFFFFFF
+        mask = 0xFFFFFFFF if types is None else sum(1 << int(t) for t in types)
+        self._check(self.lib.rd_set_composite(self.h, len(pieces), C.cast(arr, C.c_void_p), mask))
+
+    def set_external_mode(self, surrogate: bool):
+        self._check(self.lib.rd_set_external_mode(self.h, 1 if surrogate else 0))
+
+    def composite_eval(self, xyz):
+        """Exact composite on the GPU at positions [m,3] -> (grad [m,3], energy [m])."""
+        x = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        g = np.zeros_like(x)
+        e = np.zeros(len(x), dtype=np.float32)
+        self._check(self.lib.rd_composite_eval(self.h, len(x), _ptr(x), _ptr(g), _ptr(e)))
+        return g, e
+
+    def surrogate_init(self, seed: int = 1):
+        self._check(self.lib.rd_surrogate_init(self.h, int(seed)))
+
+    def surrogate_weights(self) -> np.ndarray:
+        w = np.zeros(8771, dtype=np.float32)
+        self._check(self.lib.rd_surrogate_get_weights(self.h, _ptr(w)))
+        return w
+
+    def set_surrogate_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != 8771:
+            raise ValueError("the surrogate has 8771 parameters")
+        self._check(self.lib.rd_surrogate_set_weights(self.h, _ptr(w)))
+
+    def surrogate_norm(self):
+        lo = np.zeros(3)
+        hi = np.zeros(3)
+        gs = C.c_double()
+        self._check(self.lib.rd_surrogate_get_norm(self.h, _ptr(lo), _ptr(hi), C.byref(gs)))
+        return lo, hi, gs.value
+
+    def set_surrogate_norm(self, lo, hi, gscale):
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        self._check(self.lib.rd_surrogate_set_norm(self.h, _ptr(lo), _ptr(hi), float(gscale)))
+
+    def surrogate_make_data(self, nsamples: int, seed: int = 1) -> float:
+        gs = C.c_double()
+        self._check(self.lib.rd_surrogate_make_data(self.h, int(nsamples), int(seed), C.byref(gs)))
+        return gs.value
+
+    def surrogate_train(self, steps: int, batch: int = 8192, lr: float = 3e-3, lr_min: float = 1e-5,
+                        seed: int = 1) -> np.ndarray:
+        tp = _lib.rd_train_params()
+        tp.batch, tp.steps, tp.lr, tp.lr_min = int(batch), int(steps), lr, lr_min
+        tp.beta1, tp.beta2, tp.eps, tp.seed = 0.9, 0.999, 1e-8, int(seed)
+        curve = np.zeros(int(steps), dtype=np.float32)
+        self._check(self.lib.rd_surrogate_train(self.h, C.byref(tp), _ptr(curve)))
+        return curve
+
+    def surrogate_loss_grad(self, xn, t):
+        x = np.ascontiguousarray(xn, dtype=np.float32).reshape(-1, 3)
+        tt = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 3)
+        g = np.zeros(8771, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.rd_surrogate_loss_grad(self.h, len(x), _ptr(x), _ptr(tt), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def surrogate_predict(self, xn):
+        x = np.ascontiguousarray(xn, dtype=np.float32).reshape(-1, 3)
+        y = np.zeros_like(x)
+        self._check(self.lib.rd_surrogate_predict(self.h, len(x), _ptr(x), _ptr(y)))
+        return y
+
+    def surrogate_validate(self, nsamples: int = 200_000, seed: int = 99) -> float:
+        r = C.c_double()
+        self._check(self.lib.rd_surrogate_validate(self.h, int(nsamples), int(seed), C.byref(r)))
+        return r.value
+
     def info(self) -> _lib.rd_info:
         i = _lib.rd_info()
         self._check(self.lib.rd_get_info(self.h, C.byref(i)))

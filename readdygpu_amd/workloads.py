@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .model import (ATTRACTIVE, HARMONIC, PIECEWISE_WELL, REPULSIVE, SPHERE, OrderOnePotential,
+from .model import (ATTRACTIVE, CYLINDER, DISK, HARMONIC, PIECEWISE_WELL, REPULSIVE, SPHERE, OrderOnePotential,
                     OrderTwoPotential, ParticleType, SystemSpec)
 
 
@@ -93,3 +93,43 @@ def batched_replicas(n_replicas: int = 256, seed: int = 3, nmin: int = 500, nmax
 
 
 BATCHED = dict(steps=1000, rdf_range=20.0, rdf_bins=200)
+
+
+def composite_pieces(rng: np.random.Generator, n_pieces: int, lo, hi, k: float = 5.0, rmin: float = 10.0,
+                     rmax: float = 80.0):
+    """Attractive z-normal pieces, alternating DISK and CYLINDER, centres
+    uniform in [lo, hi), radii and cylinder heights uniform in [rmin, rmax]."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    pieces = []
+    for i in range(n_pieces):
+        c = lo + rng.random(3) * (hi - lo)
+        r = float(rng.uniform(rmin, rmax))
+        hgt = float(rng.uniform(rmin, rmax))
+        kind = DISK if i % 2 == 0 else CYLINDER
+        pieces.append(OrderOnePotential(kind, ATTRACTIVE, k=k, origin=tuple(float(v) for v in c), normal=(0.0, 0.0, 1.0),
+                                        radius=r, height=hgt if kind == CYLINDER else 0.0))
+    return pieces
+
+
+def learned_slab(n: int = 2_000_000, seed: int = 4, n_pieces: int = 512, box=(2000.0, 2000.0, 400.0),
+                 cell_size: float = 0.0):
+    """2,000,000 particles of 3 species (radii 1.0/1.5/2.0 nm, D 1.0/0.5/0.25)
+    in a 2000 x 2000 x 400 nm periodic slab; harmonic repulsion k=10; dt=0.05 ns;
+    a composite external potential of 512 attractive z-normal pieces (disks and
+    cylinders alternating, centres uniform in the slab, radii and heights
+    uniform in [10, 80] nm, k=5) combined by nearest piece.
+    Returns (spec, pieces, xyz, types)."""
+    spec = SystemSpec(box_lo=(0.0, 0.0, 0.0), box_hi=tuple(float(b) for b in box), dt=0.05, seed=seed,
+                      cell_size=cell_size)
+    spec.types = [ParticleType("A", D=1.0, radius=1.0), ParticleType("B", D=0.5, radius=1.5),
+                  ParticleType("C", D=0.25, radius=2.0)]
+    spec.pot2 = [OrderTwoPotential(HARMONIC, REPULSIVE, k=10.0)]
+    rng = np.random.default_rng(seed)
+    pieces = composite_pieces(rng, n_pieces, spec.box_lo, spec.box_hi)
+    xyz = uniform_positions(rng, n, spec.box_lo, spec.box_hi)
+    types = (np.arange(n) % 3).astype(np.int32)
+    return spec, pieces, xyz, types
+
+
+LEARNED = dict(steps=500, train_samples=4_000_000)

@@ -1,0 +1,131 @@
+"""Composite external potential and the BF16 MLP surrogate on the GPU, against
+the CPU oracle.
+
+Tolerances:
+  composite energy          |e - e_ref| <= 1e-5 * max(1, |e_ref|)
+  composite gradient        |g - g_ref| <= 1e-4 * max|g_ref| + 1e-4, where the
+                            two nearest pieces differ by > 1e-3 in energy
+  MLP loss gradient         ||g - g_ref|| / ||g_ref|| <= 2e-3 against the oracle
+                            with BF16 rounding at the kernels' points, <= 3e-2
+                            against the FP64 oracle
+  MLP forward               |y - y_ref| <= 5e-3 * max|y_ref| (BF16 oracle)
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from readdygpu_amd import Simulation, SystemSpec, workloads
+from readdygpu_amd.model import HARMONIC, REPULSIVE, OrderTwoPotential, ParticleType
+
+pytestmark = pytest.mark.gpu
+
+
+def small_system(n=3000, n_pieces=16, seed=21, box=(200.0, 200.0, 100.0)):
+    spec = SystemSpec(box_lo=(0.0, 0.0, 0.0), box_hi=box, dt=0.05, seed=seed)
+    spec.types = [ParticleType("A", D=1.0, radius=1.0), ParticleType("B", D=0.5, radius=1.5)]
+    spec.pot2 = [OrderTwoPotential(HARMONIC, REPULSIVE, k=10.0)]
+    rng = np.random.default_rng(seed)
+    pieces = workloads.composite_pieces(rng, n_pieces, spec.box_lo, spec.box_hi, rmin=5.0, rmax=30.0)
+    xyz = workloads.uniform_positions(rng, n, spec.box_lo, spec.box_hi)
+    types = (np.arange(n) % 2).astype(np.int32)
+    return spec, pieces, xyz, types
+
+
+def test_composite_exact_matches_oracle():
+    spec, _, xyz, types = small_system()
+    rng = np.random.default_rng(3)
+    pieces = workloads.composite_pieces(rng, 512, spec.box_lo, spec.box_hi, rmin=5.0, rmax=30.0)
+    x = workloads.uniform_positions(rng, 20000, spec.box_lo, spec.box_hi)
+    with Simulation(spec, xyz, types) as sim:
+        sim.set_composite(pieces)
+        g, e = sim.composite_eval(x)
+    gr, er, which, gap = orc.composite(pieces, x.astype(np.float64))
+    assert np.abs(e - er).max() <= 1e-5 * max(1.0, np.abs(er).max())
+    sel = gap > 1e-3 * np.maximum(1.0, er)
+    assert sel.mean() > 0.99
+    tol = 1e-4 * np.abs(gr).max() + 1e-4
+    assert np.abs(g[sel] - gr[sel]).max() <= tol
+
+
+def test_gradients_include_the_composite():
+    spec, pieces, xyz, types = small_system()
+    o = orc.OracleSystem(spec)
+    with Simulation(spec, xyz, types) as sim:
+        g0, _ = sim.gradients()
+        sim.set_composite(pieces, types=[0])
+        g1, e1 = sim.gradients()
+        gc, ec = sim.composite_eval(xyz)
+    gp, _ = o.gradients(xyz.astype(np.float64), types, brute=True)
+    tol = 2e-5 * np.abs(gp).max() + 1e-5
+    assert np.abs(g0 - gp).max() <= tol
+    ext = np.where((types == 0)[:, None], gc, 0.0)
+    assert np.abs(g1 - (gp + ext)).max() <= tol + 1e-5 * np.abs(ext).max()
+
+
+def test_mlp_loss_gradient_matches_oracle():
+    spec, _, xyz, types = small_system()
+    rng = np.random.default_rng(8)
+    m = 1000                                    # not a multiple of 64: padded last wave
+    xn = rng.uniform(-1, 1, (m, 3)).astype(np.float32)
+    t = rng.normal(0, 1, (m, 3)).astype(np.float32)
+    with Simulation(spec, xyz, types) as sim:
+        sim.surrogate_init(seed=5)
+        w = sim.surrogate_weights().astype(np.float64)
+        loss, g = sim.surrogate_loss_grad(xn, t)
+        y = sim.surrogate_predict(xn)
+    l16, g16 = orc.mlp_loss_grad(w, xn, t, True)
+    l64, g64 = orc.mlp_loss_grad(w, xn, t, False)
+    y16 = orc.mlp_forward(w, xn, True)
+    assert abs(loss - l16) <= 2e-3 * abs(l16)
+    r16 = np.linalg.norm(g - g16) / np.linalg.norm(g16)
+    r64 = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    print(f"loss {loss:.6f} (bf16 oracle {l16:.6f}); grad rel err vs bf16 oracle {r16:.2e}, vs fp64 {r64:.2e}")
+    assert r16 <= 2e-3
+    assert r64 <= 3e-2
+    # every parameter block is exercised
+    for a, b in [(0, 192), (192, 256), (256, 4352), (4352, 4416), (4416, 8512), (8512, 8576), (8576, 8768),
+                 (8768, 8771)]:
+        blk = np.linalg.norm(g[a:b] - g16[a:b]) / max(np.linalg.norm(g16[a:b]), 1e-12)
+        assert blk <= 1e-2, (a, b, blk)
+    assert np.abs(y - y16).max() <= 5e-3 * np.abs(y16).max()
+
+
+def test_training_reduces_the_force_error():
+    spec, pieces, xyz, types = small_system(n_pieces=4)
+    with Simulation(spec, xyz, types) as sim:
+        sim.set_composite(pieces)
+        sim.surrogate_init(seed=2)
+        gs = sim.surrogate_make_data(1 << 17, seed=3)
+        assert gs > 0
+        r0 = sim.surrogate_validate(50_000, seed=77)
+        curve = sim.surrogate_train(steps=600, batch=4096, lr=3e-3, lr_min=1e-4)
+        r1 = sim.surrogate_validate(50_000, seed=77)
+    print(f"rel force RMSE before {r0:.3f} after {r1:.4f}; loss {curve[0]:.4f} -> {curve[-1]:.5f}")
+    assert np.isfinite(curve).all()
+    assert curve[-50:].mean() < 0.2 * curve[:10].mean()
+    assert r1 < 0.5 * r0
+
+
+def test_surrogate_in_the_step():
+    """The step with the surrogate adds gscale * mlp(xn) to the pair gradient,
+    and runs stably."""
+    spec, pieces, xyz, types = small_system(n_pieces=4)
+    o = orc.OracleSystem(spec)
+    with Simulation(spec, xyz, types) as sim:
+        sim.set_composite(pieces)
+        sim.surrogate_init(seed=2)
+        sim.surrogate_make_data(1 << 16, seed=3)
+        sim.surrogate_train(steps=200, batch=4096)
+        sim.set_external_mode(True)
+        g, _ = sim.gradients()
+        lo, hi, gsc = sim.surrogate_norm()
+        sc = (2.0 / (hi - lo)).astype(np.float32)
+        xn = ((xyz - lo.astype(np.float32)) * sc - np.float32(1.0)).astype(np.float32)
+        y = sim.surrogate_predict(xn)
+        sim.simulate(50)
+        x50, _ = sim.copy_pos_from_device()
+    gp, _ = o.gradients(xyz.astype(np.float64), types, brute=True)
+    ext = y.astype(np.float64) * np.float32(gsc)
+    tol = 2e-5 * np.abs(gp).max() + 1e-5 + 1e-5 * np.abs(ext).max()
+    assert np.abs(g - (gp + ext)).max() <= tol
+    assert np.isfinite(x50).all()
