@@ -71,6 +71,126 @@ def cpu_baseline(spec, xyz, types, budget_s=15.0):
     }
 
 
+def bench_handle(sim, steps, warmup):
+    """(wall ms of the graph pass, force ms, sort ms, external-pass ms of the
+    event pass) for `steps` steps after `warmup` steps."""
+    sim.simulate(warmup)
+    out = (C.c_double * 5)()
+    rc = sim.lib.rd_bench_steps(sim.h, steps, out)
+    if rc != 0:
+        raise RuntimeError(sim.lib.rd_last_error(sim.h).decode())
+    sim.simulate(0)
+    return out[0], out[2], out[3], out[4]
+
+
+def bench_batched(steps=200, warmup=10):
+    """256 replicas (log-uniform 500..50,000 particles, phi = 0.05) in one
+    handle, against the same replicas run one handle at a time."""
+    from readdygpu_amd import Simulation, workloads
+    spec, boxes, counts, xyz, types = workloads.batched_replicas()
+    n = int(counts.sum())
+    sim = Simulation(spec, xyz, types, replicas=(boxes, counts))
+    ms_tune = sim.autotune(trial_steps=5)
+    wall, force, sort, _ = bench_handle(sim, steps, warmup)
+    info = sim.info()
+    sim.close()
+    batched = n * steps / (wall / 1e3)
+    # the plain form: each replica on its own handle, all steps of one before the next
+    k = max(10, steps // 10)
+    tot_ms = 0.0
+    off = 0
+    for r, c in enumerate(counts):
+        import dataclasses
+        b = boxes[r]
+        s1 = dataclasses.replace(spec, box_lo=(b[0], b[2], b[4]), box_hi=(b[1], b[3], b[5]), stream=r)
+        one = Simulation(s1, xyz[off:off + c], types[off:off + c])
+        w1, _, _, _ = bench_handle(one, k, 2)
+        one.close()
+        tot_ms += w1
+        off += c
+    separate = n * k / (tot_ms / 1e3)
+    log(f"batched: {batched:.3e} p-steps/s ({wall / steps * 1e3:.1f} us/step), one handle per replica "
+        f"{separate:.3e}")
+    return {
+        "workload": "batched_replicas: 256 periodic boxes, N log-uniform in [500, 50000] (seed 3), phi=0.05, "
+                    "2 species, harmonic repulsion, dt=0.1 ns",
+        "n_particles": n, "n_replicas": int(len(counts)), "steps": steps,
+        "value": batched, "unit": "particle-steps/s", "ms_per_step": wall / steps,
+        "force_avg_us": force / steps * 1e3, "sort_avg_us": sort / steps * 1e3,
+        "plain_per_replica_value": separate, "speedup_vs_plain": batched / separate,
+        "cell_size_nm": info.cell_size, "autotune_us_per_step": ms_tune * 1e3,
+    }
+
+
+def rdf_l2(g_a, g_b):
+    """relative L2 deviation of g(r) curves (all type pairs and bins)"""
+    import numpy as np
+    return float(np.linalg.norm(g_a - g_b) / max(np.linalg.norm(g_b), 1e-30))
+
+
+def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000):
+    """2M particles in a 2000x2000x400 nm slab with the 512-piece composite:
+    exact external pass against the trained BF16 surrogate."""
+    import numpy as np
+    from readdygpu_amd import Simulation, workloads
+    spec, pieces, xyz, types = workloads.learned_slab()
+    n = len(types)
+    res = {"workload": "learned_slab: 2,000,000 particles, 3 species, 2000x2000x400 nm, harmonic repulsion, "
+                       "512-piece nearest-piece composite (k=5, radii 10-80 nm), dt=0.05 ns",
+           "n_particles": n, "steps": steps}
+    sim = Simulation(spec, xyz, types, rdf=(20.0, 200))
+    sim.set_composite(pieces)
+    t0 = time.perf_counter()
+    sim.surrogate_init(seed=4)
+    gs = sim.surrogate_make_data(samples, seed=4)
+    t1 = time.perf_counter()
+    curve = sim.surrogate_train(steps=train_steps, batch=8192, lr=3e-3, lr_min=1e-5, seed=4)
+    t2 = time.perf_counter()
+    rmse = sim.surrogate_validate(400_000, seed=12345)
+    res.update({"train_samples": samples, "train_steps": train_steps, "train_batch": 8192,
+                "data_s": t1 - t0, "train_s": t2 - t1, "loss_first": float(curve[0]), "loss_last": float(curve[-1]),
+                "force_rel_rmse": rmse, "gscale": gs})
+    log(f"learned: data {t1 - t0:.2f}s train {t2 - t1:.2f}s loss {curve[0]:.4f}->{curve[-1]:.5f} "
+        f"rel force RMSE {rmse:.4f}")
+    w = sim.surrogate_weights()
+    lo, hi, gsc = sim.surrogate_norm()
+    sim.autotune(trial_steps=3)
+    # exact external pass (this handle), then a fresh handle with the same
+    # start, seed and step counter running the surrogate: identical noise
+    wall_e, force_e, sort_e, ext_e = bench_handle(sim, steps, warmup)
+    sim.call_rdf_calculation()
+    g_exact = sim.normalize_rdf()
+    sim.close()
+    sim = Simulation(spec, xyz, types, rdf=(20.0, 200))
+    sim.set_composite(pieces)
+    sim.set_surrogate_weights(w)
+    sim.set_surrogate_norm(lo, hi, gsc)
+    sim.set_external_mode(True)
+    sim.autotune(trial_steps=3)
+    wall_s, force_s, sort_s, ext_s = bench_handle(sim, steps, warmup)
+    sim.call_rdf_calculation()
+    g_sur = sim.normalize_rdf()
+    sim.close()
+    outs = {}
+    for name, (w, f, so, e) in {"exact": (wall_e, force_e, sort_e, ext_e),
+                                "surrogate": (wall_s, force_s, sort_s, ext_s)}.items():
+        outs[name] = {"value": n * steps / (w / 1e3), "ms_per_step": w / steps, "ext_avg_us": e / steps * 1e3,
+                      "force_avg_us": f / steps * 1e3, "sort_avg_us": so / steps * 1e3}
+    f_alg = 17152.0
+    outs["surrogate"]["mfma"] = {"flop_alg_per_particle": f_alg,
+                                 "achieved_tflops": f_alg * n / (ext_s / steps / 1e3) / 1e12,
+                                 "peak_tflops": 2500.0}
+    outs["surrogate"]["mfma"]["frac"] = outs["surrogate"]["mfma"]["achieved_tflops"] / 2500.0
+    res.update(outs)
+    res["speedup_surrogate_vs_exact_step"] = outs["surrogate"]["value"] / outs["exact"]["value"]
+    res["speedup_surrogate_vs_exact_external"] = ext_e / max(ext_s, 1e-12)
+    res["rdf_rel_l2"] = rdf_l2(g_sur, g_exact)
+    log(f"learned: exact {outs['exact']['value']:.3e} p-steps/s (ext {outs['exact']['ext_avg_us']:.0f} us), "
+        f"surrogate {outs['surrogate']['value']:.3e} (ext {outs['surrogate']['ext_avg_us']:.0f} us), "
+        f"RDF rel L2 {res['rdf_rel_l2']:.4f}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,15 +204,27 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--fill", type=float, default=0.0)
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned workloads")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES bytes (profiler HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported", file=sys.stderr)
         sys.exit(2)
 
     from readdygpu_amd import Simulation, workloads
-    from readdygpu_amd import _lib
 
     spec, xyz, types = workloads.plain_1m(n=args.n, cell_size=args.cell)
+    if args.calibrate_copy:
+        sim = Simulation(spec, xyz[:1000], types[:1000])
+        ms = C.c_double()
+        rc = sim.lib.rd_bench_copy(sim.h, int(args.calibrate_copy), 10, C.byref(ms))
+        if rc != 0:
+            raise RuntimeError(sim.lib.rd_last_error(sim.h).decode())
+        sim.close()
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "reps": 10, "avg_ms": ms.value,
+                          "gbs": 2 * args.calibrate_copy / (ms.value / 1e3) / 1e9}))
+        return
     n = len(types)
     log(f"setting up plain_1m n={n}")
     sim = Simulation(spec, xyz, types)
@@ -101,16 +233,16 @@ def main():
     if not args.no_tune and args.kernel < 0 and args.cell == 0.0:
         ms = sim.autotune(trial_steps=10)
         log(f"autotuned: {ms * 1e3:.1f} us/step")
+    sim.simulate(args.warmup)
     info = sim.info()
     lc = sim.launch_config()
-    log(f"lattice {list(info.lattice)} cell {info.cell_size:.3f} nm; kernel {lc}; warmup {args.warmup} steps")
-    sim.simulate(args.warmup)
+    log(f"lattice {list(info.lattice)} cell {info.cell_size:.3f} nm; kernel {lc}; after {args.warmup} warmup steps")
 
-    out = (C.c_double * 4)()
+    out = (C.c_double * 5)()
     rc = sim.lib.rd_bench_steps(sim.h, args.steps, out)
     if rc != 0:
         raise RuntimeError(sim.lib.rd_last_error(sim.h).decode())
-    wall_ms, gpu_ms, force_ms, sort_ms = list(out)
+    wall_ms, gpu_ms, force_ms, sort_ms, _ = list(out)
     sim.simulate(0)
     log(f"timed {args.steps} steps: wall {wall_ms:.3f} ms, events {gpu_ms:.3f} ms, "
         f"force/update {force_ms:.3f} ms, sort {sort_ms:.3f} ms")
@@ -130,9 +262,18 @@ def main():
         "bytes_alg_per_launch": B_ALG * n,
         "sort_avg_us": sort_ms / args.steps * 1e3,
     }
+    sim.close()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(spec, xyz, types)
+    extras = {}
+    if not args.no_extras:
+        for name, fn in (("batched_replicas", bench_batched), ("learned_slab", bench_learned)):
+            try:
+                extras[name] = fn()
+            except Exception as e:            # reported, not fatal: the headline line still prints
+                log(f"{name} failed: {e}")
+                extras[name] = {"error": str(e)}
     res = {
         "metric": METRIC,
         "value": value,
@@ -158,8 +299,8 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extras": extras,
     }
-    sim.close()
     print(json.dumps(res))
 
 

@@ -16,7 +16,8 @@ extern "C" {
  * and the cell sort of every step (graph nodes carry no event timestamps).
  * out[0] = host wall time of graph launch + synchronize (ms),
  * out[1] = first event to last of pass 2 (ms), out[2] = sum of force/update kernel
- * times, out[3] = sum of cell-sort times. */
+ * times, out[3] = sum of cell-sort times, out[4] = sum of external-pass times
+ * (composite or surrogate; 0 without).  out holds 5 values. */
 int rd_bench_steps(rd_system* h, int nsteps, double* out);
 
 /* Copy `bytes` bytes device-to-device with a plain vector-copy kernel, reps

@@ -7,10 +7,6 @@
 #include <string>
 #include <vector>
 
-// shared with rd_system.hip
-int rd_host_refresh(rd_system* h);
-void rd_host_launch_force(rd_system* h);
-void rd_host_launch_sort(rd_system* h, int advance);
 
 __global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n)
 {
@@ -34,16 +30,18 @@ extern "C" int rd_bench_steps(rd_system* h, int nsteps, double* out)
     if (!h || nsteps < 1 || !out) return RD_EINVAL;
     int rc = rd_host_refresh(h);
     if (rc) return rc;
-    std::vector<hipEvent_t> ev(3 * (size_t)nsteps, nullptr);
+    std::vector<hipEvent_t> ev(4 * (size_t)nsteps, nullptr);
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
-    double tf = 0.0, ts = 0.0;
+    double tf = 0.0, ts = 0.0, te = 0.0;
+    const bool ext = h->ext_active();
     float ms = 0.0f;
     std::chrono::steady_clock::time_point t0, t1;
     for (auto& e : ev) BCK(hipEventCreate(&e));
     // pass 1: the K steps as one graph, host wall clock around launch + sync
     BCK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < nsteps; ++i) {
+        if (ext) rd_host_launch_ext(h);
         rd_host_launch_force(h);
         rd_host_launch_sort(h, 1);
     }
@@ -59,24 +57,29 @@ extern "C" int rd_bench_steps(rd_system* h, int nsteps, double* out)
     // pass 2: K more steps launched directly with events around each kernel
     // group (events recorded inside a graph have no timestamps in HIP)
     for (int i = 0; i < nsteps; ++i) {
-        BCK(hipEventRecord(ev[3 * i], h->stream));
+        BCK(hipEventRecord(ev[4 * i], h->stream));
+        if (ext) rd_host_launch_ext(h);
+        BCK(hipEventRecord(ev[4 * i + 1], h->stream));
         rd_host_launch_force(h);
-        BCK(hipEventRecord(ev[3 * i + 1], h->stream));
+        BCK(hipEventRecord(ev[4 * i + 2], h->stream));
         rd_host_launch_sort(h, 1);
-        BCK(hipEventRecord(ev[3 * i + 2], h->stream));
+        BCK(hipEventRecord(ev[4 * i + 3], h->stream));
     }
     BCK(hipStreamSynchronize(h->stream));
     h->step += (uint64_t)nsteps;
     for (int i = 0; i < nsteps; ++i) {
-        BCK(hipEventElapsedTime(&ms, ev[3 * i], ev[3 * i + 1]));
+        BCK(hipEventElapsedTime(&ms, ev[4 * i], ev[4 * i + 1]));
+        te += ms;
+        BCK(hipEventElapsedTime(&ms, ev[4 * i + 1], ev[4 * i + 2]));
         tf += ms;
-        BCK(hipEventElapsedTime(&ms, ev[3 * i + 1], ev[3 * i + 2]));
+        BCK(hipEventElapsedTime(&ms, ev[4 * i + 2], ev[4 * i + 3]));
         ts += ms;
     }
-    BCK(hipEventElapsedTime(&ms, ev[0], ev[3 * (size_t)nsteps - 1]));
+    BCK(hipEventElapsedTime(&ms, ev[0], ev[4 * (size_t)nsteps - 1]));
     out[1] = ms;
     out[2] = tf;
     out[3] = ts;
+    out[4] = te;
 done:
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);

@@ -41,8 +41,11 @@ def test_composite_exact_matches_oracle():
         g, e = sim.composite_eval(x)
     gr, er, which, gap = orc.composite(pieces, x.astype(np.float64))
     assert np.abs(e - er).max() <= 1e-5 * max(1.0, np.abs(er).max())
-    sel = gap > 1e-3 * np.maximum(1.0, er)
-    assert sel.mean() > 0.99
+    # away from switches between pieces; inside a cylinder every candidate has
+    # zero energy and zero gradient, so ties there are harmless
+    sel = (gap > 1e-3 * np.maximum(1.0, er)) | (er == 0.0)
+    assert sel.mean() > 0.99, sel.mean()
+    assert (er == 0.0).mean() < 0.9
     tol = 1e-4 * np.abs(gr).max() + 1e-4
     assert np.abs(g[sel] - gr[sel]).max() <= tol
 
