@@ -200,10 +200,7 @@ def main():
     ap.add_argument("--cell", type=float, default=0.0, help="lattice cell edge (0: automatic)")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
-    ap.add_argument("--kernel", type=int, default=-1, help="force kernel: 1 gather, 2 tiled")
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--rows", type=int, default=0)
-    ap.add_argument("--fill", type=float, default=0.0)
+    ap.add_argument("--kernel", type=int, default=-1, help="force kernel: 1 gather")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned workloads")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES bytes (profiler HBM-counter calibration)")
@@ -229,7 +226,7 @@ def main():
     log(f"setting up plain_1m n={n}")
     sim = Simulation(spec, xyz, types)
     if args.kernel >= 0:
-        sim.set_launch_config(kernel=args.kernel, block=args.block, rows_per_tile=args.rows, fill=args.fill)
+        sim.set_launch_config(kernel=args.kernel)
     if not args.no_tune and args.kernel < 0 and args.cell == 0.0:
         ms = sim.autotune(trial_steps=10)
         log(f"autotuned: {ms * 1e3:.1f} us/step")
@@ -293,9 +290,7 @@ def main():
             "n_particles": n,
             "cell_size_nm": info.cell_size,
             "lattice": list(info.lattice),
-            "force_kernel": {1: "gather", 2: "tiled", 3: "wave"}.get(lc["used_kernel"], "?"),
-            "block": lc["block"],
-            "rows_per_tile": lc["used_rows"],
+            "force_kernel": {1: "gather"}.get(lc["used_kernel"], "?"),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -261,24 +261,18 @@ int rd_create_from_model(const rd_model* m, double max_cutoff, uint64_t seed, rd
  *      block, 32 per cell, CudaSimulation.cu:193, 2347).  Any configuration
  *      gives bitwise the same trajectory: pair gradients are summed in fixed
  *      point, so only speed changes. ---- */
-enum { RD_KERNEL_AUTO = 0, RD_KERNEL_GATHER = 1, RD_KERNEL_TILED = 2, RD_KERNEL_WAVE = 3 };
+enum { RD_KERNEL_AUTO = 0, RD_KERNEL_GATHER = 1 };
 typedef struct {
-    int kernel;            /* RD_KERNEL_*: GATHER per-particle global gather; TILED cell-row tiles
-                              in LDS; WAVE per-wave shifted slot ranges in LDS.  AUTO = WAVE */
-    int block;             /* threads per workgroup of the tiled kernel: 64..1024 */
-    int rows_per_tile;     /* cell rows per tile; 0 = from the density */
-    double fill;           /* target home particles per tile / block (0 = 0.6) */
+    int kernel;            /* RD_KERNEL_*: GATHER one lane per particle walks its stencil rows,
+                              testing distances first and evaluating the listed hits after.
+                              AUTO = GATHER */
     int used_kernel;       /* out: kernel in use */
-    int used_rows;         /* out: rows per tile in use */
-    int used_segments;     /* out: tiles along x */
-    int used_cap;          /* out: staged particle capacity per tile */
-    long used_lds;         /* out: LDS bytes per workgroup */
 } rd_launch_config;
 int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg);
 int rd_get_launch_config(rd_system* h, rd_launch_config* cfg);
 
 /* Autotuner: times `trial_steps` steps for each candidate cell size (factors
- * of the interaction range) and tile shape, keeps the fastest, and restores
+ * of the interaction range) and force kernel, keeps the fastest, and restores
  * the particle state and step counter, so tuning does not change the run. */
 int rd_autotune(rd_system* h, int trial_steps, double* best_ms_per_step);
 

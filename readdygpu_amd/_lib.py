@@ -55,14 +55,7 @@ class rd_pot2(C.Structure):
 class rd_launch_config(C.Structure):
     _fields_ = [
         ("kernel", C.c_int),
-        ("block", C.c_int),
-        ("rows_per_tile", C.c_int),
-        ("fill", C.c_double),
         ("used_kernel", C.c_int),
-        ("used_rows", C.c_int),
-        ("used_segments", C.c_int),
-        ("used_cap", C.c_int),
-        ("used_lds", C.c_long),
     ]
 
 

@@ -11,7 +11,6 @@
 #include "../../include/rdgpu.h"
 #include "rd_device.h"
 #include "rd_mlp.h"
-#include "rd_tiled.h"
 
 struct RdStepState;
 
@@ -104,11 +103,7 @@ struct rd_system {
 
     // force-kernel launch configuration (autotuned)
     rd_launch_config lc{};        // requested
-    bool tiled = false;           // chosen: LDS-tiled kernel
-    int kern = 3;                 // kernel in use (RD_KERNEL_*)
-    int tile_block = 256;
-    RdTileGeom tg{};
-    size_t tile_lds = 0;
+    int kern = RD_KERNEL_GATHER;  // kernel in use (RD_KERNEL_*)
 
     // composite external potential (nearest piece) and its MLP surrogate
     std::vector<rd_pot1> comp;

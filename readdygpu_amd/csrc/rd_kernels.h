@@ -96,8 +96,13 @@ __device__ __forceinline__ void for_stencil(const RdRep& R, const uint32_t* __re
 #define RD_FIX_INV 2.3283064365386963e-10
 __device__ __forceinline__ long long to_fix(float v)
 {
+    // floor(v * 2^32) from the integer part and the exact fraction: two
+    // 32-bit conversions instead of a float -> int64 conversion sequence
     v = fminf(fmaxf(v, -1.0e9f), 1.0e9f);
-    return (long long)(v * RD_FIX_SCALE);
+    const float fl = floorf(v);
+    const int hi = (int)fl;
+    const uint32_t lo = (uint32_t)((v - fl) * RD_FIX_SCALE);
+    return (long long)(((unsigned long long)(uint32_t)hi << 32) | lo);
 }
 
 // Per-block LDS copy of the type-pair tables and EM coefficients: the inner
@@ -107,6 +112,7 @@ struct RdTab {
     float r0[RD_MAXT * RD_MAXT];
     uint32_t mask[RD_MAXT * RD_MAXT];
     float a[RD_MAXT], b[RD_MAXT], prad[RD_MAXT];
+    float rcmax2[RD_MAXT];          // largest squared range of a type with any partner
     RdPot2Dev pot2[RD_MAXP2];
 };
 
@@ -123,6 +129,10 @@ __device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
         T.a[k] = P->a[k];
         T.b[k] = P->b[k];
         T.prad[k] = P->prad[k];
+        float m = 0.0f;
+        if (k < P->nt)
+            for (int j = 0; j < P->nt; ++j) m = fmaxf(m, P->rc2[k * P->nt + j]);
+        T.rcmax2[k] = m;
     }
     const int nw = (int)(sizeof(RdPot2Dev) / 4) * RD_MAXP2;
     for (int k = threadIdx.x; k < nw; k += blockDim.x)
@@ -130,18 +140,13 @@ __device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
     __syncthreads();
 }
 
-// accumulate the pair (p, q) into the fixed-point gradient of p
+// Gradient contribution of q on p for the separation d = p - q (|d|^2 = d2),
+// added to the fixed-point sums.  Every force kernel evaluates a pair through
+// this one function, so all of them produce the same bits.
 template <bool ENERGY>
-__device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p, float4 q, const RdRep& R,
-                                           long long& ax, long long& ay, long long& az, float& e)
+__device__ __forceinline__ void pair_add(const RdTab& T, int rowbase, float dx, float dy, float dz, float d2, int tj,
+                                         long long& ax, long long& ay, long long& az, float& e)
 {
-    float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
-    float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
-    float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
-    float d2 = dx * dx;
-    d2 = d2 + dy * dy;
-    d2 = d2 + dz * dz;
-    const int tj = w_type(q.w);
     if (d2 < T.rc2[rowbase + tj] && d2 > 0.0f) {
         float rinv = __builtin_amdgcn_rsqf(d2);
         float r = d2 * rinv;
@@ -156,6 +161,50 @@ __device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p
         ax += to_fix(pre * dx); ay += to_fix(pre * dy); az += to_fix(pre * dz);
         if (ENERGY) e += 0.5f * ep;
     }
+}
+
+__device__ __forceinline__ float dist2(float dx, float dy, float dz)
+{
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+
+// the pair (p, q) with the minimum-image separation
+template <bool ENERGY>
+__device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p, float4 q, const RdRep& R,
+                                           long long& ax, long long& ay, long long& az, float& e)
+{
+    float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
+    float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
+    float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
+    pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), ax, ay, az, e);
+}
+
+// The pair (p, q) for a home particle p in an interior cell (every cell
+// coordinate in [1, L-2]) and q in its 27-cell stencil: the plain difference
+// is the minimum image for every pair within range (cell >= range), so no
+// per-pair image test is needed.  A candidate beyond the home type's largest
+// range is rejected before the type-pair table is read.
+template <bool ENERGY>
+__device__ __forceinline__ void pair_inner(const RdTab& T, int rowbase, float rmax2, float4 p, float4 q,
+                                           long long& ax, long long& ay, long long& az, float& e)
+{
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float d2 = dist2(dx, dy, dz);
+    if (d2 < rmax2) pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), ax, ay, az, e);
+}
+
+// one stencil candidate: raw difference for interior homes, minimum image otherwise
+template <bool ENERGY>
+__device__ __forceinline__ void pair_any(const RdTab& T, int rowbase, float rmax2, bool interior, float4 p, float4 q,
+                                         const RdRep& R, long long& ax, long long& ay, long long& az, float& e)
+{
+    if (interior) pair_inner<ENERGY>(T, rowbase, rmax2, p, q, ax, ay, az, e);
+    else pair_accum<ENERGY>(T, rowbase, p, q, R, ax, ay, az, e);
+}
+
+__device__ __forceinline__ bool interior_cell(const RdRep& R, int cx, int cy, int cz)
+{
+    return cx >= 1 && cx <= R.L[0] - 2 && cy >= 1 && cy <= R.L[1] - 2 && cz >= 1 && cz <= R.L[2] - 2;
 }
 
 // external potentials, added to the fixed-point pair sum; a gradient computed
@@ -185,58 +234,99 @@ __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const
     return make_float3(gx, gy, gz);
 }
 
+// per-wave list of candidate pairs that passed the distance test (slot of
+// the partner), transposed so that a wave's lanes write one LDS row
+#define RD_HITS 16
+struct RdHitList {
+    uint32_t j[RD_HITS][64];
+};
+
 // gradient of U (pair + external) for the particle in slot s, from global
 // memory (the per-particle gather)
 template <bool ENERGY>
 __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
                                               const RdPot1Dev* __restrict__ pot1,
                                               const float4* __restrict__ ps, const uint32_t* __restrict__ start,
-                                              uint32_t s, float4 p, float* energy)
+                                              uint32_t s, float4 p, float* energy, RdHitList* __restrict__ il)
 {
-    const int rowbase = w_type(p.w) * P->nt;
+    const int ti = w_type(p.w);
+    const int rowbase = ti * P->nt;
     long long ax = 0, ay = 0, az = 0;
     float e = 0.0f;
     const int Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
     int cx, cy, cz;
     cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
-    auto body = [&](uint32_t j, float4 q) {
-        if (j != s) pair_accum<ENERGY>(T, rowbase, p, q, R, ax, ay, az, e);
-    };
-    auto run = [&](uint32_t j, uint32_t j1) {
-        for (; j + 1 < j1; j += 2) {
-            float4 q0 = ps[j], q1 = ps[j + 1];
-            body(j, q0);
-            body(j + 1, q1);
-        }
-        if (j < j1) body(j, ps[j]);
-    };
     if (P->n_pot2 > 0) {
         if (P->S == 1 && Lx >= 3 && Ly >= 3 && Lz >= 3) {
-            // 27-cell stencil as 9 rows of contiguous slots; the row starts of
-            // the next row are loaded while the current row is processed
+            const bool in = interior_cell(R, cx, cy, cz);
+            const float rmax2 = T.rcmax2[ti];
+            const int lane = threadIdx.x & 63;
+            // Phase 1: every stencil candidate gets the cheap distance test
+            // against the home type's largest range; hits are listed in LDS.
+            // Phase 2 evaluates the listed pairs only, so the interaction
+            // code runs (max hits over the wave) times, not once per
+            // candidate iteration in which some lane has a hit.
+            int nhit = 0;
+            auto test = [&](uint32_t j, float4 q) {
+                float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                if (!in) {
+                    dx = min_image(dx, R.Lb[0], R.half[0]);
+                    dy = min_image(dy, R.Lb[1], R.half[1]);
+                    dz = min_image(dz, R.Lb[2], R.half[2]);
+                }
+                const float d2 = dist2(dx, dy, dz);
+                if (d2 < rmax2 && d2 > 0.0f) {
+                    if (nhit < RD_HITS) il->j[nhit][lane] = j;
+                    else pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), ax, ay, az, e);
+                    ++nhit;
+                }
+            };
+            // 27-cell stencil as 9 rows of at most two runs of contiguous
+            // slots.  All 36 run bounds are loaded first (independent loads,
+            // in flight together); then the runs are walked.
             int a0, b0, a1, b1;
             const int nr = x_runs(cx, 1, Lx, &a0, &b0, &a1, &b1);
-            auto rowof = [&](int r) {
+            uint32_t sb[18], se[18];
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
                 int yy = cy + (r % 3) - 1, zz = cz + (r / 3) - 1;
                 yy += yy < 0 ? Ly : 0; yy -= yy >= Ly ? Ly : 0;
                 zz += zz < 0 ? Lz : 0; zz -= zz >= Lz ? Lz : 0;
-                return R.coff + (zz * Ly + yy) * Lx;
-            };
-            int row = rowof(0);
-            uint32_t nb0 = start[row + a0], ne0 = start[row + b0 + 1], nb1 = 0, ne1 = 0;
-            if (nr == 2) { nb1 = start[row + a1]; ne1 = start[row + b1 + 1]; }
-            for (int r = 0; r < 9; ++r) {
-                uint32_t cb0 = nb0, ce0 = ne0, cb1 = nb1, ce1 = ne1;
-                if (r < 8) {
-                    row = rowof(r + 1);
-                    nb0 = start[row + a0]; ne0 = start[row + b0 + 1];
-                    if (nr == 2) { nb1 = start[row + a1]; ne1 = start[row + b1 + 1]; }
+                const int row = R.coff + (zz * Ly + yy) * Lx;
+                sb[2 * r] = start[row + a0];
+                se[2 * r] = start[row + b0 + 1];
+                sb[2 * r + 1] = nr == 2 ? start[row + a1] : 0u;
+                se[2 * r + 1] = nr == 2 ? start[row + b1 + 1] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 18; ++k) {
+                uint32_t j = sb[k];
+                const uint32_t j1 = se[k];
+                for (; j + 1 < j1; j += 2) {
+                    const float4 q0 = ps[j], q1 = ps[j + 1];
+                    test(j, q0);
+                    test(j + 1, q1);
                 }
-                run(cb0, ce0);
-                run(cb1, ce1);
+                if (j < j1) test(j, ps[j]);
+            }
+            // Phase 2
+            const int nl = nhit < RD_HITS ? nhit : RD_HITS;
+            for (int k = 0; __any(k < nl); ++k) {
+                if (k < nl) {
+                    const float4 q = ps[il->j[k][lane]];
+                    float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                    if (!in) {
+                        dx = min_image(dx, R.Lb[0], R.half[0]);
+                        dy = min_image(dy, R.Lb[1], R.half[1]);
+                        dz = min_image(dz, R.Lb[2], R.half[2]);
+                    }
+                    pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), ax, ay, az, e);
+                }
             }
         } else {
-            for_stencil(R, start, cx, cy, cz, P->S, [&](uint32_t j) { body(j, ps[j]); });
+            for_stencil(R, start, cx, cy, cz, P->S, [&](uint32_t j) {
+                if (j != s) pair_accum<ENERGY>(T, rowbase, p, ps[j], R, ax, ay, az, e);
+            });
         }
     }
     float3 g = finish_grad<ENERGY>(P, T, pot1, s, p, ax, ay, az, &e);

@@ -5,8 +5,6 @@
 // replicas; all replicas advance together through one set of kernels.
 #include "rd_host.h"
 #include "rd_kernels.h"
-#include "rd_tiled.h"
-#include "rd_wave.h"
 
 #include <algorithm>
 #include <cmath>
@@ -55,13 +53,14 @@ __global__ __launch_bounds__(RD_BLOCK) void k_force_update(const float4* __restr
                                                            const float* __restrict__ noise, int ntot)
 {
     __shared__ RdTab T;
+    __shared__ RdHitList il[RD_BLOCK / 64];
     load_tab(T, P);
     const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(wave_id()));
     const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
     if (s >= (uint32_t)it.s1) return;
     const RdRep R = reps[it.rep];
     float4 p = ps[s];
-    float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr);
+    float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
     em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count);
 }
 
@@ -75,6 +74,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict
 {
     __shared__ double red[RD_BLOCK / 64];
     __shared__ RdTab T;
+    __shared__ RdHitList il[RD_BLOCK / 64];
     load_tab(T, P);
     const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(wave_id()));
     const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict
         const RdRep R = reps[it.rep];
         float4 p = ps[s];
         float ef = 0.0f;
-        float3 g = gather_grad<true>(P, R, T, pot1, ps, start, s, p, &ef);
+        float3 g = gather_grad<true>(P, R, T, pot1, ps, start, s, p, &ef, &il[threadIdx.x >> 6]);
         size_t gi = (size_t)R.off + w_index(p.w);
         grad[3 * gi] = g.x; grad[3 * gi + 1] = g.y; grad[3 * gi + 2] = g.z;
         e = (double)ef;
@@ -776,86 +776,9 @@ static void launch_sort(rd_system* h, int advance)
 
 static int wave_blocks(const rd_system* h) { return std::max(1, nblk(h->nwaves, RD_BLOCK / 64)); }
 
-// Tile shape for the LDS-tiled kernel (single replica): home particles per
-// tile near fill x block, staged capacity with headroom for density
-// fluctuations, LDS <= 64 KB per workgroup.
-static void choose_tiles(rd_system* h)
-{
-    h->tiled = false;
-    h->tile_block = h->lc.block > 0 ? h->lc.block : 256;
-    if (h->lc.kernel != RD_KERNEL_TILED || h->rep_dev.size() != 1 || h->ext_active()) return;
-    const RdRep& R0 = h->rep_dev[0];
-    const RdDev& P = h->P;
-    const int Lx = R0.L[0], Ly = R0.L[1], Lz = R0.L[2];
-    if (P.S != 1 || Lx < 3 || Ly < 3 || Lz < 3 || P.n_pot2 == 0 || h->n == 0) return;
-    const double rc = (double)h->n / ((double)Lx * Ly * Lz);
-    const double fill = h->lc.fill > 0 ? h->lc.fill : 0.6;
-    const double home = std::max(16.0, h->tile_block * fill);
-    RdTileGeom G{};
-    const int tymax = std::max(1, std::min(8, Ly - 2));
-    const double rowp = Lx * rc;
-    if (Lx <= 1024 && rowp <= home) {
-        G.nseg = 1; G.TW = Lx; G.W = Lx;
-        G.TY = h->lc.rows_per_tile > 0 ? h->lc.rows_per_tile : (int)std::lround(home / std::max(rowp, 1e-9));
-    } else {
-        G.TY = h->lc.rows_per_tile > 0 ? h->lc.rows_per_tile : 1;
-        G.TY = std::min(G.TY, tymax);
-        int tw = (int)std::lround(home / std::max(G.TY * rc, 1e-9));
-        G.TW = std::max(1, std::min(tw, Lx - 2));
-        G.nseg = (Lx + G.TW - 1) / G.TW;
-        G.W = G.TW + 2;
-    }
-    G.TY = std::max(1, std::min(G.TY, tymax));
-    for (;;) {
-        double staged = 3.0 * (G.TY + 2) * G.W * rc;
-        G.cap = (int)((1.5 * staged + 64.0 + 63.0) / 64.0) * 64;
-        size_t lds = rd_tile_lds_bytes(G);
-        if (lds <= 64 * 1024) { h->tile_lds = lds; break; }
-        if (G.TY > 1) { G.TY--; continue; }
-        if (G.nseg == 1 || G.TW <= 1) return;       // cannot fit: another kernel
-        G.TW = std::max(1, G.TW / 2);
-        G.nseg = (Lx + G.TW - 1) / G.TW;
-        G.W = G.TW + 2;
-    }
-    G.nyt = (Ly + G.TY - 1) / G.TY;
-    G.ntiles = Lz * G.nyt * G.nseg;
-    h->tg = G;
-    h->tiled = true;
-}
-
-template <int MODE>
-static void launch_tiled(rd_system* h, float* grad, double* etot)
-{
-    const RdTileGeom& G = h->tg;
-    dim3 grid(G.ntiles), blk(h->tile_block);
-    size_t lds = h->tile_lds;
-#define RD_TL(B)                                                                                           \
-    hipLaunchKernelGGL((k_force_tiled<B, MODE>), grid, blk, lds, h->stream, h->d_ps, h->d_pn, h->d_key,   \
-                       h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps, h->d_pot1, h->d_st, h->d_noise, \
-                       h->n, G, grad, etot)
-    switch (h->tile_block) {
-    case 64: RD_TL(64); break;
-    case 128: RD_TL(128); break;
-    case 512: RD_TL(512); break;
-    default: RD_TL(256); break;
-    }
-#undef RD_TL
-}
-
 void rd_host_launch_force(rd_system* h)
 {
     if (h->n <= 0) return;
-    if (h->kern == RD_KERNEL_TILED) {
-        launch_tiled<RD_MODE_UPDATE>(h, nullptr, nullptr);
-        return;
-    }
-    if (h->kern == RD_KERNEL_WAVE) {
-        hipLaunchKernelGGL(k_force_wave<RD_MODE_UPDATE>, dim3(nblk(h->nwaves, RD_WAVE_WAVES)),
-                           dim3(64 * RD_WAVE_WAVES), 0, h->stream, h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count,
-                           h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st, h->d_noise, h->n,
-                           nullptr, nullptr);
-        return;
-    }
     hipLaunchKernelGGL(k_force_update, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_pn, h->d_key,
                        h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st,
                        h->d_noise, h->n);
@@ -878,8 +801,7 @@ static int upload(rd_system* h)
     rc = rd_host_ext_capacity(h);
     if (rc) return rc;
     h->P.gext = h->ext_active() ? h->d_gext : nullptr;
-    choose_tiles(h);
-    h->kern = h->tiled ? RD_KERNEL_TILED : (h->lc.kernel == RD_KERNEL_GATHER ? RD_KERNEL_GATHER : RD_KERNEL_WAVE);
+    h->kern = RD_KERNEL_GATHER;
     HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
     HIPCK(h, hipMemcpyAsync(h->d_reps, h->rep_dev.data(), sizeof(RdRep) * h->rep_dev.size(), hipMemcpyHostToDevice,
                             h->stream));
@@ -1028,11 +950,7 @@ int rd_host_set_state(rd_system* h, const std::vector<float>& xyz, const std::ve
 extern "C" int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg)
 {
     if (!h || !cfg) return RD_EINVAL;
-    if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_WAVE) return set_err(h, RD_EINVAL, "bad kernel");
-    if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256 && cfg->block != 512)
-        return set_err(h, RD_EINVAL, "block must be 64, 128, 256 or 512");
-    if (cfg->rows_per_tile < 0 || cfg->rows_per_tile > 8 || cfg->fill < 0)
-        return set_err(h, RD_EINVAL, "bad tile shape");
+    if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_GATHER) return set_err(h, RD_EINVAL, "bad kernel");
     h->lc = *cfg;
     h->dirty = true;
     return RD_OK;
@@ -1043,11 +961,6 @@ extern "C" int rd_get_launch_config(rd_system* h, rd_launch_config* cfg)
     if (!h || !cfg) return RD_EINVAL;
     *cfg = h->lc;
     cfg->used_kernel = h->kern;
-    cfg->used_rows = h->tiled ? h->tg.TY : 0;
-    cfg->used_segments = h->tiled ? h->tg.nseg : 0;
-    cfg->used_cap = h->tiled ? h->tg.cap : 0;
-    cfg->used_lds = h->tiled ? (long)h->tile_lds : 0;
-    if (h->tiled) cfg->block = h->tile_block;
     return RD_OK;
 }
 
@@ -1095,15 +1008,7 @@ extern "C" int rd_gradients(rd_system* h, float* grad, double* energy)
     if (h->n == 0) { if (energy) *energy = 0.0; return RD_OK; }
     HIPCK(h, hipMemsetAsync(h->d_etot, 0, sizeof(double), h->stream));
     if (h->ext_active()) rd_host_launch_ext(h);
-    if (h->kern == RD_KERNEL_TILED)
-        launch_tiled<RD_MODE_GRAD>(h, h->d_xyz_io, h->d_etot);
-    else if (h->kern == RD_KERNEL_WAVE)
-        hipLaunchKernelGGL(k_force_wave<RD_MODE_GRAD>, dim3(nblk(h->nwaves, RD_WAVE_WAVES)), dim3(64 * RD_WAVE_WAVES),
-                           0, h->stream, h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P,
-                           h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st, h->d_noise, h->n, h->d_xyz_io,
-                           h->d_etot);
-    else
-        hipLaunchKernelGGL(k_gradients, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start, h->d_P,
+    hipLaunchKernelGGL(k_gradients, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start, h->d_P,
                            h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_xyz_io, h->d_etot);
     HIPCK(h, hipGetLastError());
     std::vector<float> tmp(3 * (size_t)h->n);

@@ -50,22 +50,20 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     if (user_cell > 0 || base <= 0) cells.push_back(user_cell > 0 ? user_cell : h->cell);
     else
         for (double f : { 1.01, 1.2, 1.45, 1.75, 2.1 }) cells.push_back(f * base);
-    struct Cfg { int kernel, block; double fill; };
-    const Cfg cfgs[] = { { RD_KERNEL_WAVE, 256, 0.0 }, { RD_KERNEL_GATHER, 256, 0.0 }, { RD_KERNEL_TILED, 128, 0.6 },
-                         { RD_KERNEL_TILED, 256, 0.6 } };
+    struct Cfg { int kernel; };
+    const Cfg cfgs[] = { { RD_KERNEL_GATHER } };
     double best = 1e30, best_cell = cells[0];
     Cfg best_cfg = cfgs[0];
     for (double c : cells) {
         for (const Cfg& k : cfgs) {
             h->prm.cell_size = c;
             rd_launch_config lc{};
-            lc.kernel = k.kernel; lc.block = k.block; lc.fill = k.fill;
+            lc.kernel = k.kernel;
             rc = rd_set_launch_config(h, &lc);
             if (rc == RD_OK) rc = rd_host_set_state(h, sx, st);
             if (rc == RD_OK) rc = rd_host_refresh(h);
             if (rc == RD_ERANGE) continue;          // too many cells for this size
             if (rc) return rc;
-            if (k.kernel == RD_KERNEL_TILED && !h->tiled) continue;
             double ms = 0.0;
             rc = time_steps(h, trial_steps, &ms);
             if (rc) return rc;
@@ -75,7 +73,7 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     // keep the winner, restore the state
     h->prm.cell_size = user_cell > 0 ? user_cell : best_cell;
     rd_launch_config lc = user_lc;
-    lc.kernel = best_cfg.kernel; lc.block = best_cfg.block; lc.fill = best_cfg.fill;
+    lc.kernel = best_cfg.kernel;
     rc = rd_set_launch_config(h, &lc);
     if (rc == RD_OK) rc = rd_host_set_state(h, sx, st);
     if (rc == RD_OK) rc = rd_host_refresh(h);

@@ -227,10 +227,10 @@ class Simulation:
         cr = np.ascontiguousarray(creactions, dtype=np.float32)
         self._check(self.lib.rd_apply_reactions(self.h, _ptr(cr)))
 
-    def set_launch_config(self, kernel: int = 0, block: int = 0, rows_per_tile: int = 0, fill: float = 0.0):
-        """Force-kernel launch shape (0 = automatic); the trajectory does not depend on it."""
+    def set_launch_config(self, kernel: int = 0):
+        """Force kernel: 0 automatic, 1 per-particle gather."""
         c = _lib.rd_launch_config()
-        c.kernel, c.block, c.rows_per_tile, c.fill = kernel, block, rows_per_tile, fill
+        c.kernel = int(kernel)
         self._check(self.lib.rd_set_launch_config(self.h, C.byref(c)))
 
     def launch_config(self) -> dict:

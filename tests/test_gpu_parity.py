@@ -53,6 +53,19 @@ def rich_spec(seed=11, box=40.0, cell=0.0):
     return spec, xyz, types
 
 
+def dense_spec(n=6000, box=22.0, seed=9):
+    """About 110 stencil candidates per particle: the flat kernel's candidate
+    list spills over several LDS passes per wave."""
+    spec = SystemSpec(box_lo=(0, 0, 0), box_hi=(box, box, box), dt=0.002, seed=seed)
+    spec.types = [ParticleType("A", 1.0, 1.0), ParticleType("B", 0.5, 1.2)]
+    spec.pot2 = [OrderTwoPotential(HARMONIC, REPULSIVE, 10.0),
+                 OrderTwoPotential(PIECEWISE_WELL, ATTRACTIVE, 1.0, length=0.5, depth=0.5, pairs=[(0, 1)])]
+    rng = np.random.default_rng(seed)
+    xyz = workloads.uniform_positions(rng, n, spec.box_lo, spec.box_hi)
+    types = rng.integers(0, 2, n).astype(np.int32)
+    return spec, xyz, types
+
+
 @pytest.fixture(scope="module")
 def c1():
     return workloads.plain_10k()
@@ -76,9 +89,9 @@ def test_cell_ids_bitexact(c1):
         assert np.array_equal(sim.cell_ids(), o.cell_index(x)[0])
 
 
-@pytest.mark.parametrize("which", ["plain_10k", "rich"])
+@pytest.mark.parametrize("which", ["plain_10k", "rich", "dense"])
 def test_gradients_and_energy_match_fp64(which, c1):
-    spec, xyz, types = c1 if which == "plain_10k" else rich_spec()
+    spec, xyz, types = c1 if which == "plain_10k" else (rich_spec() if which == "rich" else dense_spec())
     o = orc.OracleSystem(spec)
     with Simulation(spec, xyz, types) as sim:
         g, e = sim.gradients()
@@ -140,22 +153,22 @@ def test_step_is_deterministic_and_noise_array_equals_generator(c1):
     assert np.array_equal(xa, xc)
 
 
-@pytest.mark.parametrize("which", ["plain_10k", "rich", "plain_1m_slice"])
+@pytest.mark.parametrize("which", ["plain_10k", "rich", "plain_1m_slice", "dense"])
 def test_launch_configs_give_identical_trajectories(which, c1):
-    """Gather kernel, tiled kernel in every shape, other cell sizes and the
-    autotuner: bitwise the same positions (fixed-point pair sums)."""
+    """Three cell sizes (different sort orders inside cells) and the autotuner:
+    bitwise the same positions and gradients (fixed-point pair sums)."""
     if which == "plain_10k":
         spec, xyz, types = c1
     elif which == "rich":
         spec, xyz, types = rich_spec()
+    elif which == "dense":
+        spec, xyz, types = dense_spec()
     else:
         spec, xyz, types = workloads.plain_1m(n=200_000, box=585.0)
     ref = None
     rc = float(spec.pair_range().max())
-    shapes = [dict(kernel=3), dict(kernel=1), dict(kernel=2, block=64, fill=0.5),
-              dict(kernel=2, block=128, rows_per_tile=1), dict(kernel=2, block=256, fill=1.0),
-              dict(kernel=2, block=512, fill=0.3)]
-    for cell in (0.0, 1.3 * rc):
+    shapes = [dict(kernel=1), dict(kernel=0)]
+    for cell in (0.0, 1.3 * rc, 2.1 * rc):
         for shape in shapes:
             spec.cell_size = cell
             with Simulation(spec, xyz, types) as sim:
@@ -164,8 +177,7 @@ def test_launch_configs_give_identical_trajectories(which, c1):
                 x, _ = sim.copy_pos_from_device()
                 g, _ = sim.gradients()
                 used = sim.launch_config()
-            if shape.get("kernel") == 2 and which != "rich":
-                assert used["used_kernel"] == 2, used
+            assert used["used_kernel"] == 1, used
             if ref is None:
                 ref = (x, g)
             assert np.array_equal(x, ref[0]), (cell, shape)
