@@ -524,39 +524,89 @@ __global__ void k_mlp_step_inc(RdTrainCtl* ctl) { ctl->step += 1; }
 // ===========================================================================
 // Inference
 // ===========================================================================
-// Persistent waves: the forward weights live in registers (A fragments, 72
-// VGPRs), layer-1 weights and biases in LDS; each wave loops over chunks of
-// 64 samples.  io.load(c, xn) / io.store(c, lane, g) move one chunk.
+// Persistent waves: the forward weights (A fragments, 18 KB) and layer-1
+// weights and biases are staged in LDS per workgroup.  A chunk of
+// 64 samples is pushed through the network one sample tile (16 samples) at a
+// time, so only one tile's activations are live (as packed BF16 B operands):
+// few enough registers for several waves per SIMD, which overlap one wave's
+// SiLU work (VALU) with another's MFMAs.  Outputs of tile tj land in lanes
+// 0..15 (sample 16 tj + lane), which store them directly.
+struct RdMlpFwdLds {
+    rd_bf16x8 w2[4][2][64], w3[4][2][64], w4[2][64];
+    float w1[192], b1[64], b2[64], b3[64], b4[4];
+};
+
 template <class IO>
 __device__ __forceinline__ void mlp_infer_waves(const RdMlpPack* __restrict__ pk_g, long nchunks, IO& io)
 {
-    __shared__ float w1[192], b1[64], b2[64], b3[64], b4[4];
-    for (int i = threadIdx.x; i < 192; i += blockDim.x) w1[i] = pk_g->w1[i];
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) { b1[i] = pk_g->b1[i]; b2[i] = pk_g->b2[i]; b3[i] = pk_g->b3[i]; }
-    if (threadIdx.x < 4) b4[threadIdx.x] = pk_g->b4[threadIdx.x];
+    __shared__ RdMlpFwdLds L;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(pk_g);      // w2, w3, w4 lead the pack
+        uint4* dst = reinterpret_cast<uint4*>(&L);
+        const int n16 = (int)(sizeof(rd_bf16x8) * (2 * 4 * 2 * 64 + 2 * 64) / 16);
+        for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+        for (int i = threadIdx.x; i < 192; i += blockDim.x) L.w1[i] = pk_g->w1[i];
+        for (int i = threadIdx.x; i < 64; i += blockDim.x) { L.b1[i] = pk_g->b1[i]; L.b2[i] = pk_g->b2[i]; L.b3[i] = pk_g->b3[i]; }
+        if (threadIdx.x < 4) L.b4[threadIdx.x] = pk_g->b4[threadIdx.x];
+    }
     __syncthreads();
+    const float* w1 = L.w1;
+    const float* b1 = L.b1;
+    const float* b2 = L.b2;
+    const float* b3 = L.b3;
+    const float* b4 = L.b4;
     const int lane = threadIdx.x & 63;
-    rd_bf16x8 w2[4][2], w3[4][2], w4[2];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int kb = 0; kb < 2; ++kb) { w2[a][kb] = pk_g->w2[a][kb][lane]; w3[a][kb] = pk_g->w3[a][kb][lane]; }
-    w4[0] = pk_g->w4[0][lane];
-    w4[1] = pk_g->w4[1][lane];
+    const int j = lane & 15, g = lane >> 4;
     const long wpb = blockDim.x >> 6;
     const long stride = (long)gridDim.x * wpb;
     for (long c = (long)xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); c < nchunks; c += stride) {
-        float xn[4][3];
-        io.load(c, lane, xn);
-        float z[4][4][4], h[4][4][4], y[4][3];
-        rd_mlp_layer1(w1, b1, xn, z, lane);
-        rd_mlp_silu_all(z, h);
-        rd_mlp_layer64([&](int to, int kb) { return w2[to][kb]; }, b2, h, z, lane);
-        rd_mlp_silu_all(z, h);
-        rd_mlp_layer64([&](int to, int kb) { return w3[to][kb]; }, b3, h, z, lane);
-        rd_mlp_silu_all(z, h);
-        rd_mlp_layer4([&](int kb) { return w4[kb]; }, b4, h, y, lane);
-        io.store(c, lane, rd_mlp_gather_out(y, lane));
+#pragma unroll 1
+        for (int tj = 0; tj < 4; ++tj) {
+            float xn[3];
+            io.load(c * 64 + tj * 16 + j, xn);
+            // layer 1 (FP32, VALU) -> SiLU -> B operands
+            rd_f32x4 z[4];
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int f = ti * 16 + 4 * g + r;
+                    float v = w1[3 * f] * xn[0];
+                    v = v + w1[3 * f + 1] * xn[1];
+                    v = v + w1[3 * f + 2] * xn[2];
+                    z[ti][r] = v + b1[f];
+                }
+            rd_bf16x8 hb[2];
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+                for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
+            // layers 2 and 3 (MFMA) -> SiLU
+#pragma unroll
+            for (int layer = 0; layer < 2; ++layer) {
+                const float* bb = layer == 0 ? b2 : b3;
+#pragma unroll
+                for (int to = 0; to < 4; ++to) {
+                    rd_f32x4 acc;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[r] = bb[to * 16 + 4 * g + r];
+                    acc = RD_MFMA(layer == 0 ? L.w2[to][0][lane] : L.w3[to][0][lane], hb[0], acc);
+                    acc = RD_MFMA(layer == 0 ? L.w2[to][1][lane] : L.w3[to][1][lane], hb[1], acc);
+                    z[to] = acc;
+                }
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
+            }
+            // layer 4 (MFMA, rows 0..2 of 16)
+            rd_f32x4 acc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = (g == 0 && r < 3) ? b4[r] : 0.0f;
+            acc = RD_MFMA(L.w4[0][lane], hb[0], acc);
+            acc = RD_MFMA(L.w4[1][lane], hb[1], acc);
+            if (g == 0) io.store(c * 64 + tj * 16 + j, make_float3(acc[0], acc[1], acc[2]));
+        }
     }
 }
 
@@ -567,20 +617,15 @@ struct SlotIO {
     int n;
     RdMlpNorm nm;
     uint32_t type_mask;
-    __device__ void load(long c, int lane, float (&xn)[4][3]) const
+    __device__ void load(long s, float (&xn)[3]) const
     {
-#pragma unroll
-        for (int tj = 0; tj < 4; ++tj) {
-            long s = c * 64 + tj * 16 + (lane & 15);
-            float4 p = s < n ? ps[s] : make_float4(nm.lo[0], nm.lo[1], nm.lo[2], 0.f);
-            xn[tj][0] = norm_coord(p.x, nm.lo[0], nm.sc[0]);
-            xn[tj][1] = norm_coord(p.y, nm.lo[1], nm.sc[1]);
-            xn[tj][2] = norm_coord(p.z, nm.lo[2], nm.sc[2]);
-        }
+        float4 p = s < n ? ps[s] : make_float4(nm.lo[0], nm.lo[1], nm.lo[2], 0.f);
+        xn[0] = norm_coord(p.x, nm.lo[0], nm.sc[0]);
+        xn[1] = norm_coord(p.y, nm.lo[1], nm.sc[1]);
+        xn[2] = norm_coord(p.z, nm.lo[2], nm.sc[2]);
     }
-    __device__ void store(long c, int lane, float3 y) const
+    __device__ void store(long s, float3 y) const
     {
-        long s = c * 64 + lane;
         if (s >= n) return;
         float4 p = ps[s];
         bool on = (type_mask >> w_type(p.w)) & 1u;
@@ -600,18 +645,13 @@ struct PointIO {
     const float4* __restrict__ xn;
     float4* __restrict__ out;
     long m;
-    __device__ void load(long c, int lane, float (&x)[4][3]) const
+    __device__ void load(long i, float (&x)[3]) const
     {
-#pragma unroll
-        for (int tj = 0; tj < 4; ++tj) {
-            long i = c * 64 + tj * 16 + (lane & 15);
-            float4 p = i < m ? xn[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-            x[tj][0] = p.x; x[tj][1] = p.y; x[tj][2] = p.z;
-        }
+        float4 p = i < m ? xn[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        x[0] = p.x; x[1] = p.y; x[2] = p.z;
     }
-    __device__ void store(long c, int lane, float3 y) const
+    __device__ void store(long i, float3 y) const
     {
-        long i = c * 64 + lane;
         if (i < m) out[i] = make_float4(y.x, y.y, y.z, 0.f);
     }
 };

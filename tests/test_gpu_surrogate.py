@@ -132,3 +132,35 @@ def test_surrogate_in_the_step():
     tol = 2e-5 * np.abs(gp).max() + 1e-5 + 1e-5 * np.abs(ext).max()
     assert np.abs(g - (gp + ext)).max() <= tol
     assert np.isfinite(x50).all()
+
+
+def test_single_piece_surrogate_meets_the_accuracy_gates():
+    """Where the geometry is learnable by the 3-64-64-64-3 network (one disk
+    in the 2000 x 2000 x 400 nm slab), the trained surrogate reaches force
+    RMSE <= 1% on fresh samples, and a run with it keeps the RDF within 0.5%
+    (relative L2) of the exact run from the same start with the same noise."""
+    spec, _, xyz, types = workloads.learned_slab(n=100_000)
+    pieces = workloads.composite_pieces(np.random.default_rng(4), 1, spec.box_lo, spec.box_hi)
+    with Simulation(spec, xyz, types, rdf=(20.0, 100)) as ex:
+        ex.set_composite(pieces)
+        ex.surrogate_init(seed=4)
+        ex.surrogate_make_data(1 << 21, seed=4)
+        ex.surrogate_train(steps=3000, batch=8192, lr=3e-3, lr_min=1e-5)
+        rmse = ex.surrogate_validate(200_000, seed=99)
+        w = ex.surrogate_weights()
+        lo, hi, gs = ex.surrogate_norm()
+        ex.simulate(100)
+        ex.call_rdf_calculation()
+        g_exact = ex.normalize_rdf()
+    with Simulation(spec, xyz, types, rdf=(20.0, 100)) as su:
+        su.set_composite(pieces)
+        su.set_surrogate_weights(w)
+        su.set_surrogate_norm(lo, hi, gs)
+        su.set_external_mode(True)
+        su.simulate(100)
+        su.call_rdf_calculation()
+        g_sur = su.normalize_rdf()
+    dev = np.linalg.norm(g_sur - g_exact) / np.linalg.norm(g_exact)
+    print(f"single piece: force rel RMSE {rmse:.4f}, RDF rel L2 {dev:.4f}")
+    assert rmse <= 0.01
+    assert dev <= 0.005
