@@ -239,6 +239,99 @@ def bench_membrane(steps=200, frames=20):
             "passed": bool(rmse <= 0.01 and dev <= 0.005)}
 
 
+def bench_cytosol(steps=100, warmup=10, rdf_steps=200, frames=10, plain_steps=20, train_steps=3000):
+    """Combined workload: 64 replicas (5.06M particles, phi = 0.3, 8 species,
+    two wells, A+B candidates) with a 256-disk membrane acting on the two
+    largest species.  Batched + surrogate against batched exact (the same
+    replicas, same start and noise: bitwise what each replica gives alone)
+    and against the plain form (one handle per replica, exact membrane)."""
+    import dataclasses
+    import numpy as np
+    from readdygpu_amd import Simulation, workloads
+    spec, boxes, counts, xyz, types, pieces, mtypes = workloads.crowded_cytosol()
+    n = int(counts.sum())
+    lmax = float(boxes[:, 1].max())
+    res = {"workload": "crowded_cytosol: 64 replicas, N log-uniform in [20000, 200000] (seed 5), phi=0.3, 8 species, "
+                       "2 wells, A+B candidates at 3 nm, 256-disk membrane on species 6-7, dt=0.01 ns",
+           "n_particles": n, "n_replicas": int(len(counts)), "steps": steps}
+
+    def handle(surrogate, w=None, norm=None, sp=spec):
+        sim = Simulation(sp, xyz, types, replicas=(boxes, counts), rdf=(20.0, 100))
+        sim.set_composite(pieces, types=mtypes)
+        if surrogate:
+            sim.set_surrogate_weights(w)
+            sim.set_surrogate_norm(*norm)
+            sim.set_external_mode(True)
+        return sim
+
+    sim = handle(False)
+    sim.surrogate_init(seed=5)
+    sim.set_surrogate_norm((0.0, 0.0, 0.0), (lmax, lmax, lmax), 1.0)
+    sim.surrogate_make_data(4_000_000, seed=5)
+    sim.surrogate_train(steps=train_steps, batch=8192, lr=3e-3, lr_min=1e-5, seed=5)
+    res["force_rel_rmse"] = sim.surrogate_validate(400_000, seed=55)
+    w = sim.surrogate_weights()
+    norm = sim.surrogate_norm()
+    out = {}
+    g = {}
+    for mode in ("exact", "surrogate"):
+        if mode == "surrogate":
+            sim = handle(True, w, norm)
+        sim.autotune(trial_steps=2)
+        wall, force, sort, ext = bench_handle(sim, steps, warmup)
+        out[mode] = {"value": n * steps / (wall / 1e3), "ms_per_step": wall / steps,
+                     "ext_avg_us": ext / steps * 1e3, "force_avg_us": force / steps * 1e3,
+                     "sort_avg_us": sort / steps * 1e3}
+        out[mode]["candidates_ab"] = int(len(sim.reaction_candidates()))
+        sim.close()
+        # RDF from the common start (fresh handle, same noise)
+        sim = handle(mode == "surrogate", w, norm)
+        sim.simulate(rdf_steps)
+        for _ in range(frames):
+            sim.simulate(10)
+            sim.call_rdf_calculation()
+        g[mode] = sim.normalize_rdf()
+        sim.close()
+    # statistical floor: the exact run with another noise seed
+    sim = handle(False, sp=dataclasses.replace(spec, seed=spec.seed + 1000))
+    sim.simulate(rdf_steps)
+    for _ in range(frames):
+        sim.simulate(10)
+        sim.call_rdf_calculation()
+    g["exact_seed2"] = sim.normalize_rdf()
+    sim.close()
+
+    def rel(a, b):
+        per = [float(np.linalg.norm(g[a][r] - g[b][r]) / max(np.linalg.norm(g[b][r]), 1e-30))
+               for r in range(len(counts))]
+        pa = (g[a] * counts[:, None, None, None]).sum(0)
+        pb = (g[b] * counts[:, None, None, None]).sum(0)
+        return {"mean": float(np.mean(per)), "max": float(np.max(per)),
+                "pooled": float(np.linalg.norm(pa - pb) / np.linalg.norm(pb))}
+    res.update(out)
+    res["rdf_rel_l2_surrogate_vs_exact"] = rel("surrogate", "exact")
+    res["rdf_rel_l2_exact_other_seed_vs_exact"] = rel("exact_seed2", "exact")
+    # plain form: each replica on its own handle with the exact membrane
+    tot = 0.0
+    off = 0
+    for r, c in enumerate(counts):
+        b = boxes[r]
+        s1 = dataclasses.replace(spec, box_lo=(b[0], b[2], b[4]), box_hi=(b[1], b[3], b[5]), stream=r)
+        one = Simulation(s1, xyz[off:off + c], types[off:off + c])
+        one.set_composite(pieces, types=mtypes)
+        w1, _, _, _ = bench_handle(one, plain_steps, 2)
+        one.close()
+        tot += w1
+        off += c
+    res["plain_exact_per_replica_value"] = n * plain_steps / (tot / 1e3)
+    res["speedup_batched_surrogate_vs_plain"] = out["surrogate"]["value"] / res["plain_exact_per_replica_value"]
+    res["speedup_batched_surrogate_vs_batched_exact"] = out["surrogate"]["value"] / out["exact"]["value"]
+    log(f"cytosol: batched exact {out['exact']['value']:.3e}, batched surrogate {out['surrogate']['value']:.3e}, "
+        f"plain exact {res['plain_exact_per_replica_value']:.3e} p-steps/s; force RMSE {res['force_rel_rmse']:.4f}; "
+        f"RDF rel L2 surrogate {res['rdf_rel_l2_surrogate_vs_exact']}, other seed {res['rdf_rel_l2_exact_other_seed_vs_exact']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,7 +343,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
     ap.add_argument("--kernel", type=int, default=-1, help="force kernel: 1 gather")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned workloads")
-    ap.add_argument("--only", choices=["batched_replicas", "learned_slab", "learned_membrane"], default=None,
+    ap.add_argument("--only", choices=["batched_replicas", "learned_slab", "learned_membrane", "crowded_cytosol"],
+                    default=None,
                     help="run only this extra workload and print its JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES bytes (profiler HBM-counter calibration)")
@@ -263,7 +357,7 @@ def main():
 
     if args.only:
         fn = {"batched_replicas": bench_batched, "learned_slab": bench_learned,
-              "learned_membrane": bench_membrane}[args.only]
+              "learned_membrane": bench_membrane, "crowded_cytosol": bench_cytosol}[args.only]
         print(json.dumps(fn()))
         return
     spec, xyz, types = workloads.plain_1m(n=args.n, cell_size=args.cell)
@@ -321,7 +415,7 @@ def main():
     extras = {}
     if not args.no_extras:
         for name, fn in (("batched_replicas", bench_batched), ("learned_slab", bench_learned),
-                         ("learned_membrane", bench_membrane)):
+                         ("learned_membrane", bench_membrane), ("crowded_cytosol", bench_cytosol)):
             try:
                 extras[name] = fn()
             except Exception as e:            # reported, not fatal: the headline line still prints

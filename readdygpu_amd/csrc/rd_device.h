@@ -64,6 +64,7 @@ struct RdDev {
     uint32_t p2mask[RD_MAXT * RD_MAXT];
     RdPot2Dev pot2[RD_MAXP2];
     const float4* gext;   // external gradient per slot (x,y,z, energy) from a separate pass, or null
+    uint32_t ext_mask;    // types that gext applies to (other slots of gext are not written)
 };
 
 // packed particle word: low 6 bits type, high 26 bits particle index

@@ -18,6 +18,7 @@ struct rd_system {
     rd_params prm{};
     std::string err;
     int dev = 0;
+    int ncu = 1;                     // compute units of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -111,6 +112,7 @@ struct rd_system {
     int npieces = 0;
     void* d_pieces = nullptr;
     int ext_mode = RD_EXT_EXACT;
+    int ext_blocks[2] = { 0, 0 };    // resident workgroups of the exact / surrogate slot passes (0: not yet known)
     float4* d_gext = nullptr;        // external gradient per sorted slot
     int gext_cap = 0;
     float *d_master = nullptr, *d_mom = nullptr, *d_vel = nullptr;

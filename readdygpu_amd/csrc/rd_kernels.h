@@ -32,6 +32,20 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// number of set bits of a wave ballot below this lane
+__device__ __forceinline__ int lanes_below(uint64_t b)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
+// orders a wave's LDS writes before its following LDS reads by other lanes
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // x-range of a stencil row as at most two runs of contiguous cells
 __device__ __forceinline__ int x_runs(int cx, int S, int L, int* a0, int* b0, int* a1, int* b1)
 {
@@ -218,12 +232,12 @@ __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const
     float gy = (float)((double)ay * RD_FIX_INV);
     float gz = (float)((double)az * RD_FIX_INV);
     const float4* __restrict__ gext = P->gext;
-    if (gext) {
+    const int ti = w_type(p.w);
+    if (gext && ((P->ext_mask >> ti) & 1u)) {
         const float4 q = gext[s];
         gx = gx + q.x; gy = gy + q.y; gz = gz + q.z;
         if (ENERGY) *e += q.w;
     }
-    const int ti = w_type(p.w);
     const int np1 = P->n_pot1;
     const float prad = T.prad[ti];
     for (int k = 0; k < np1; ++k) {

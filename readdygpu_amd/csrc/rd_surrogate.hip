@@ -28,8 +28,7 @@ struct RdPiece {
     float hh;          // half height (0 for a disk: its energy along the axis is s^2)
     float k;
     int zaxis;         // axis is +z: cheaper projection
-    float pad[2];
-};
+};                     // 40 bytes: 512 pieces and a 2 KB queue leave room for 7 workgroups per CU
 
 __device__ __forceinline__ float piece_energy(const RdPiece& p, float x, float y, float z)
 {
@@ -83,21 +82,66 @@ __device__ __forceinline__ float4 composite_at(const RdPiece* __restrict__ pc, i
     return make_float4(g.x, g.y, g.z, best);
 }
 
-// exact pass over sorted slots: gext[s] for particles whose type is in the mask
-// (piece table staged in dynamic LDS: np * sizeof(RdPiece) bytes)
+// exact pass over sorted slots: gext[s] for the particles whose type is in the
+// mask (other slots are left alone: the force kernel reads gext only for
+// those types).  One round of resident workgroups takes chunks of 256 slots
+// in turn (selected particles gather in some cells, e.g. on a membrane, so
+// contiguous shares would be unbalanced), and compacts the selected ones
+// into an LDS queue; whenever 256 are queued, every thread evaluates one, so
+// no lane idles on an unselected particle.
+// Piece table in dynamic LDS: np * sizeof(RdPiece) bytes.
 __global__ __launch_bounds__(256) void k_composite_slots(const float4* __restrict__ ps, int n,
                                                          const RdPiece* __restrict__ pieces, int np,
                                                          uint32_t type_mask, float4* __restrict__ gext)
 {
     extern __shared__ RdPiece pc[];
+    __shared__ uint32_t q[512];
+    __shared__ int wcnt[4];
+    for (int k = threadIdx.x; k < np; k += 256) pc[k] = pieces[k];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    auto eval = [&](uint32_t s) {
+        float4 p = ps[s];
+        gext[s] = composite_at(pc, np, p.x, p.y, p.z);
+    };
+    int qn = 0;                       // queued entries (same in every thread)
+    for (long base = (long)xcd_block(blockIdx.x, gridDim.x) * 256; base < n; base += (long)gridDim.x * 256) {
+        const long s = base + tid;
+        const bool on = s < n && ((type_mask >> w_type(ps[s].w)) & 1u);
+        const uint64_t b = __ballot(on);
+        if (lane == 0) wcnt[w] = __popcll(b);
+        __syncthreads();
+        int off = qn, tot = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            off += k < w ? wcnt[k] : 0;
+            tot += wcnt[k];
+        }
+        if (on) q[off + lanes_below(b)] = (uint32_t)s;      // qn < 256, tot <= 256
+        qn += tot;
+        __syncthreads();
+        if (qn >= 256) {
+            eval(q[tid]);
+            __syncthreads();
+            if (tid < qn - 256) q[tid] = q[256 + tid];
+            qn -= 256;
+            __syncthreads();
+        }
+    }
+    if (tid < qn) eval(q[tid]);
+}
+
+// exact pass when the composite acts on every type: one slot per thread
+__global__ __launch_bounds__(256) void k_composite_all(const float4* __restrict__ ps, int n,
+                                                       const RdPiece* __restrict__ pieces, int np,
+                                                       float4* __restrict__ gext)
+{
+    extern __shared__ RdPiece pc[];
     for (int k = threadIdx.x; k < np; k += 256) pc[k] = pieces[k];
     __syncthreads();
-    int s = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long s = (long)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (s >= n) return;
-    float4 p = ps[s];
-    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-    if ((type_mask >> w_type(p.w)) & 1u) r = composite_at(pc, np, p.x, p.y, p.z);
-    gext[s] = r;
+    const float4 p = ps[s];
+    gext[s] = composite_at(pc, np, p.x, p.y, p.z);
 }
 
 // exact evaluation at given points (float4 input, .w ignored)
@@ -525,141 +569,142 @@ __global__ void k_mlp_step_inc(RdTrainCtl* ctl) { ctl->step += 1; }
 // Inference
 // ===========================================================================
 // Persistent waves: the forward weights (A fragments, 18 KB) and layer-1
-// weights and biases are staged in LDS per workgroup.  A chunk of
-// 64 samples is pushed through the network one sample tile (16 samples) at a
-// time, so only one tile's activations are live (as packed BF16 B operands):
-// few enough registers for several waves per SIMD, which overlap one wave's
-// SiLU work (VALU) with another's MFMAs.  Outputs of tile tj land in lanes
-// 0..15 (sample 16 tj + lane), which store them directly.
+// weights and biases are staged in LDS per workgroup; the compiler keeps the
+// fragments a lane uses in registers across tiles (about 180 VGPRs: 2 waves
+// per SIMD, which overlap one wave's SiLU work on the VALU with the other's
+// MFMAs).  Samples go through the network one sample tile (16 samples) at a
+// time, so only one tile's activations are live, as packed BF16 B operands.
+// Lane l holds sample l % 16 of the tile; outputs land in lanes 0..15.
 struct RdMlpFwdLds {
     rd_bf16x8 w2[4][2][64], w3[4][2][64], w4[2][64];
     float w1[192], b1[64], b2[64], b3[64], b4[4];
 };
 
-template <class IO>
-__device__ __forceinline__ void mlp_infer_waves(const RdMlpPack* __restrict__ pk_g, long nchunks, IO& io)
+__device__ __forceinline__ void mlp_stage(RdMlpFwdLds& L, const RdMlpPack* __restrict__ pk_g)
 {
-    __shared__ RdMlpFwdLds L;
-    {
-        const uint4* src = reinterpret_cast<const uint4*>(pk_g);      // w2, w3, w4 lead the pack
-        uint4* dst = reinterpret_cast<uint4*>(&L);
-        const int n16 = (int)(sizeof(rd_bf16x8) * (2 * 4 * 2 * 64 + 2 * 64) / 16);
-        for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
-        for (int i = threadIdx.x; i < 192; i += blockDim.x) L.w1[i] = pk_g->w1[i];
-        for (int i = threadIdx.x; i < 64; i += blockDim.x) { L.b1[i] = pk_g->b1[i]; L.b2[i] = pk_g->b2[i]; L.b3[i] = pk_g->b3[i]; }
-        if (threadIdx.x < 4) L.b4[threadIdx.x] = pk_g->b4[threadIdx.x];
-    }
-    __syncthreads();
-    const float* w1 = L.w1;
-    const float* b1 = L.b1;
-    const float* b2 = L.b2;
-    const float* b3 = L.b3;
-    const float* b4 = L.b4;
+    const uint4* src = reinterpret_cast<const uint4*>(pk_g);      // w2, w3, w4 lead the pack
+    uint4* dst = reinterpret_cast<uint4*>(&L);
+    const int n16 = (int)(sizeof(rd_bf16x8) * (2 * 4 * 2 * 64 + 2 * 64) / 16);
+    for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < 192; i += blockDim.x) L.w1[i] = pk_g->w1[i];
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) { L.b1[i] = pk_g->b1[i]; L.b2[i] = pk_g->b2[i]; L.b3[i] = pk_g->b3[i]; }
+    if (threadIdx.x < 4) L.b4[threadIdx.x] = pk_g->b4[threadIdx.x];
+}
+
+// network output for this lane's sample of the tile (valid in lanes 0..15)
+__device__ __forceinline__ float3 mlp_tile(const RdMlpFwdLds& L, const float (&xn)[3])
+{
     const int lane = threadIdx.x & 63;
-    const int j = lane & 15, g = lane >> 4;
-    const long wpb = blockDim.x >> 6;
-    const long stride = (long)gridDim.x * wpb;
-    for (long c = (long)xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); c < nchunks; c += stride) {
-#pragma unroll 1
-        for (int tj = 0; tj < 4; ++tj) {
-            float xn[3];
-            io.load(c * 64 + tj * 16 + j, xn);
-            // layer 1 (FP32, VALU) -> SiLU -> B operands
-            rd_f32x4 z[4];
+    const int g = lane >> 4;
+    // layer 1 (FP32, VALU) -> SiLU -> B operands
+    rd_f32x4 z[4];
 #pragma unroll
-            for (int ti = 0; ti < 4; ++ti)
+    for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int f = ti * 16 + 4 * g + r;
-                    float v = w1[3 * f] * xn[0];
-                    v = v + w1[3 * f + 1] * xn[1];
-                    v = v + w1[3 * f + 2] * xn[2];
-                    z[ti][r] = v + b1[f];
-                }
-            rd_bf16x8 hb[2];
+        for (int r = 0; r < 4; ++r) {
+            const int f = ti * 16 + 4 * g + r;
+            float v = L.w1[3 * f] * xn[0];
+            v = v + L.w1[3 * f + 1] * xn[1];
+            v = v + L.w1[3 * f + 2] * xn[2];
+            z[ti][r] = v + L.b1[f];
+        }
+    rd_bf16x8 hb[2];
 #pragma unroll
-            for (int kb = 0; kb < 2; ++kb)
+    for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-                for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
-            // layers 2 and 3 (MFMA) -> SiLU
+        for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
+    // layers 2 and 3 (MFMA) -> SiLU
 #pragma unroll
-            for (int layer = 0; layer < 2; ++layer) {
-                const float* bb = layer == 0 ? b2 : b3;
+    for (int layer = 0; layer < 2; ++layer) {
+        const float* bb = layer == 0 ? L.b2 : L.b3;
 #pragma unroll
-                for (int to = 0; to < 4; ++to) {
-                    rd_f32x4 acc;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) acc[r] = bb[to * 16 + 4 * g + r];
-                    acc = RD_MFMA(layer == 0 ? L.w2[to][0][lane] : L.w3[to][0][lane], hb[0], acc);
-                    acc = RD_MFMA(layer == 0 ? L.w2[to][1][lane] : L.w3[to][1][lane], hb[1], acc);
-                    z[to] = acc;
-                }
-#pragma unroll
-                for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-                    for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
-            }
-            // layer 4 (MFMA, rows 0..2 of 16)
+        for (int to = 0; to < 4; ++to) {
             rd_f32x4 acc;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[r] = (g == 0 && r < 3) ? b4[r] : 0.0f;
-            acc = RD_MFMA(L.w4[0][lane], hb[0], acc);
-            acc = RD_MFMA(L.w4[1][lane], hb[1], acc);
-            if (g == 0) io.store(c * 64 + tj * 16 + j, make_float3(acc[0], acc[1], acc[2]));
+            for (int r = 0; r < 4; ++r) acc[r] = bb[to * 16 + 4 * g + r];
+            acc = RD_MFMA(layer == 0 ? L.w2[to][0][lane] : L.w3[to][0][lane], hb[0], acc);
+            acc = RD_MFMA(layer == 0 ? L.w2[to][1][lane] : L.w3[to][1][lane], hb[1], acc);
+            z[to] = acc;
         }
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
     }
+    // layer 4 (MFMA, rows 0..2 of 16)
+    rd_f32x4 acc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = (g == 0 && r < 3) ? L.b4[r] : 0.0f;
+    acc = RD_MFMA(L.w4[0][lane], hb[0], acc);
+    acc = RD_MFMA(L.w4[1][lane], hb[1], acc);
+    return make_float3(acc[0], acc[1], acc[2]);
 }
 
 // surrogate pass over sorted slots: gext[s] = gscale * mlp(normalised position)
-struct SlotIO {
-    const float4* __restrict__ ps;
-    float4* __restrict__ gext;
-    int n;
-    RdMlpNorm nm;
-    uint32_t type_mask;
-    __device__ void load(long s, float (&xn)[3]) const
-    {
-        float4 p = s < n ? ps[s] : make_float4(nm.lo[0], nm.lo[1], nm.lo[2], 0.f);
-        xn[0] = norm_coord(p.x, nm.lo[0], nm.sc[0]);
-        xn[1] = norm_coord(p.y, nm.lo[1], nm.sc[1]);
-        xn[2] = norm_coord(p.z, nm.lo[2], nm.sc[2]);
-    }
-    __device__ void store(long s, float3 y) const
-    {
-        if (s >= n) return;
-        float4 p = ps[s];
-        bool on = (type_mask >> w_type(p.w)) & 1u;
-        gext[s] = on ? make_float4(y.x * nm.gscale, y.y * nm.gscale, y.z * nm.gscale, 0.f)
-                     : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-};
-
-__global__ __launch_bounds__(256) void k_mlp_slots(const RdMlpPack* __restrict__ pk, SlotIO io)
+// for the particles whose type is in the mask (other slots are not written).
+// One round of resident waves takes chunks of 64 slots in turn; a wave
+// appends the selected slots of its chunk to a ring queue in LDS and runs a
+// tile whenever 16 are queued; the last partial tile runs at the end.
+#define RD_MLP_RING 128
+__global__ __launch_bounds__(256) void k_mlp_slots(const RdMlpPack* __restrict__ pk, const float4* __restrict__ ps,
+                                                   int n, RdMlpNorm nm, uint32_t type_mask,
+                                                   float4* __restrict__ gext)
 {
-    long nchunks = ((long)io.n + 63) / 64;
-    mlp_infer_waves(pk, nchunks, io);
+    __shared__ RdMlpFwdLds L;
+    __shared__ uint32_t ring[4][RD_MLP_RING];
+    mlp_stage(L, pk);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, j = lane & 15;
+    uint32_t* q = ring[threadIdx.x >> 6];
+    auto tile = [&](uint32_t head, int cnt) {            // cnt in 1..16 entries from head
+        const bool ok = j < cnt;
+        const uint32_t s = ok ? q[(head + j) & (RD_MLP_RING - 1)] : 0u;
+        const float4 p = ps[s];
+        float xn[3] = { norm_coord(p.x, nm.lo[0], nm.sc[0]), norm_coord(p.y, nm.lo[1], nm.sc[1]),
+                        norm_coord(p.z, nm.lo[2], nm.sc[2]) };
+        const float3 y = mlp_tile(L, xn);
+        if (lane < 16 && ok)
+            gext[s] = make_float4(y.x * nm.gscale, y.y * nm.gscale, y.z * nm.gscale, 0.f);
+    };
+    const long nchunks = ((long)n + 63) / 64;
+    const long wpb = blockDim.x >> 6;
+    const long stride = (long)gridDim.x * wpb;
+    uint32_t head = 0, tail = 0;                         // tail - head <= 15 between chunks
+    for (long c = (long)xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); c < nchunks; c += stride) {
+        const long s = c * 64 + lane;
+        const bool on = s < n && ((type_mask >> w_type(ps[s].w)) & 1u);
+        const uint64_t b = __ballot(on);
+        if (on) q[(tail + lanes_below(b)) & (RD_MLP_RING - 1)] = (uint32_t)s;
+        tail += (uint32_t)__popcll(b);
+        wave_lds_sync();
+#pragma unroll 1
+        while (tail - head >= 16) {
+            tile(head, 16);
+            head += 16;
+        }
+        wave_lds_sync();
+    }
+    if (tail != head) tile(head, (int)(tail - head));
 }
 
 // inference at normalised points (float4 xn) -> raw network output (float4)
-struct PointIO {
-    const float4* __restrict__ xn;
-    float4* __restrict__ out;
-    long m;
-    __device__ void load(long i, float (&x)[3]) const
-    {
-        float4 p = i < m ? xn[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        x[0] = p.x; x[1] = p.y; x[2] = p.z;
-    }
-    __device__ void store(long i, float3 y) const
-    {
-        if (i < m) out[i] = make_float4(y.x, y.y, y.z, 0.f);
-    }
-};
-
-__global__ __launch_bounds__(256) void k_mlp_points(const RdMlpPack* __restrict__ pk, PointIO io)
+__global__ __launch_bounds__(256) void k_mlp_points(const RdMlpPack* __restrict__ pk, const float4* __restrict__ xin,
+                                                    long m, float4* __restrict__ out)
 {
-    long nchunks = (io.m + 63) / 64;
-    mlp_infer_waves(pk, nchunks, io);
+    __shared__ RdMlpFwdLds L;
+    mlp_stage(L, pk);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, j = lane & 15;
+    const long ntiles = (m + 15) / 16;
+    const long wpb = blockDim.x >> 6;
+    const long stride = (long)gridDim.x * wpb;
+    for (long t = (long)xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6); t < ntiles; t += stride) {
+        const long i = t * 16 + j;
+        const float4 p = i < m ? xin[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float xn[3] = { p.x, p.y, p.z };
+        const float3 y = mlp_tile(L, xn);
+        if (lane < 16 && i < m) out[i] = make_float4(y.x, y.y, y.z, 0.f);
+    }
 }
 
 // ===========================================================================
@@ -693,28 +738,38 @@ void rd_host_free_surrogate(rd_system* h)
     h->gext_cap = 0; h->partial_rows = 0; h->nsamples = 0; h->loss_hist_cap = 0;
 }
 
+// persistent grid of k_mlp_points: one tile of 16 points per wave and round
+static inline unsigned mlp_point_blocks(long m) { return (unsigned)std::min(((m + 15) / 16 + 3) / 4, 2048L); }
+
 // external pass of the step (sorted slots -> gext)
 void rd_host_launch_ext(rd_system* h)
 {
     if (h->n <= 0) return;
+    const uint32_t all_types = h->P.nt >= 32 ? 0xffffffffu : (1u << h->P.nt) - 1u;
     if (h->ext_mode == RD_EXT_SURROGATE) {
-        SlotIO io;
-        io.ps = h->d_ps; io.gext = h->d_gext; io.n = h->n; io.nm = h->mlp_norm; io.type_mask = h->comp_mask;
-        long nchunks = ((long)h->n + 63) / 64;
-        long blocks = (nchunks + 3) / 4;
-        if (blocks > 2048) blocks = 2048;
-        hipLaunchKernelGGL(k_mlp_slots, dim3((unsigned)blocks), dim3(256), 0, h->stream,
-                           (const RdMlpPack*)h->d_pack, io);
+        unsigned blocks = std::min(h->ext_blocks[1] > 0 ? (unsigned)h->ext_blocks[1] : 2048u, nb256(h->n));
+        hipLaunchKernelGGL(k_mlp_slots, dim3(blocks), dim3(256), 0, h->stream, (const RdMlpPack*)h->d_pack,
+                           (const float4*)h->d_ps, h->n, h->mlp_norm, h->comp_mask, h->d_gext);
+    } else if ((h->comp_mask & all_types) == all_types) {
+        hipLaunchKernelGGL(k_composite_all, dim3(nb256(h->n)), dim3(256), pieces_lds(h), h->stream, h->d_ps, h->n,
+                           (const RdPiece*)h->d_pieces, h->npieces, h->d_gext);
     } else {
-        hipLaunchKernelGGL(k_composite_slots, dim3(nb256(h->n)), dim3(256), pieces_lds(h), h->stream, h->d_ps, h->n,
+        unsigned blocks = std::min(h->ext_blocks[0] > 0 ? (unsigned)h->ext_blocks[0] : 2048u, nb256(h->n));
+        hipLaunchKernelGGL(k_composite_slots, dim3(blocks), dim3(256), pieces_lds(h), h->stream, h->d_ps, h->n,
                            (const RdPiece*)h->d_pieces, h->npieces, h->comp_mask, h->d_gext);
     }
 }
 
-// gext buffer sized for the current particle count
+// gext buffer sized for the current particle count; resident workgroups of
+// the slot passes for the current piece count
 int rd_host_ext_capacity(rd_system* h)
 {
     if (!h->ext_active()) return RD_OK;
+    int per = 0;
+    SHIPCK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_composite_slots, 256, pieces_lds(h)));
+    h->ext_blocks[0] = std::max(per, 1) * h->ncu;
+    SHIPCK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_mlp_slots, 256, 0));
+    h->ext_blocks[1] = std::max(per, 1) * h->ncu;
     int need = h->n > 0 ? h->n : 1;
     if (need > h->gext_cap) {
         if (h->d_gext) (void)hipFree(h->d_gext);
@@ -748,6 +803,7 @@ extern "C" int rd_set_composite(rd_system* h, int n_pieces, const rd_pot1* piece
     if (!attr) {
         const int bytes = (int)(sizeof(RdPiece) * RD_PIECES_MAX);
         SHIPCK(h, hipFuncSetAttribute((const void*)k_composite_slots, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        SHIPCK(h, hipFuncSetAttribute((const void*)k_composite_all, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         SHIPCK(h, hipFuncSetAttribute((const void*)k_composite_points, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         SHIPCK(h, hipFuncSetAttribute((const void*)k_sur_data, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         attr = true;
@@ -1081,11 +1137,8 @@ extern "C" int rd_surrogate_predict(rd_system* h, long m, const float* xn, float
     hipError_t e = hipMalloc(&dout, sizeof(float4) * m);
     if (e == hipSuccess) e = hipMemcpyAsync(dx, hx.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) {
-        PointIO io;
-        io.xn = dx; io.out = dout; io.m = m;
-        long blocks = ((m + 63) / 64 + 3) / 4;
-        if (blocks > 2048) blocks = 2048;
-        hipLaunchKernelGGL(k_mlp_points, dim3((unsigned)blocks), dim3(256), 0, h->stream, (const RdMlpPack*)h->d_pack, io);
+        hipLaunchKernelGGL(k_mlp_points, dim3(mlp_point_blocks(m)), dim3(256), 0, h->stream,
+                           (const RdMlpPack*)h->d_pack, (const float4*)dx, m, dout);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(ho.data(), dout, sizeof(float4) * m, hipMemcpyDeviceToHost, h->stream);
@@ -1115,11 +1168,8 @@ extern "C" int rd_surrogate_validate(rd_system* h, long m, uint64_t seed, double
     for (int d = 0; d < 3; ++d) span[d] = (float)(h->norm_hi[d] - nm.lo[d]);
     hipLaunchKernelGGL(k_sur_data, dim3(nb256(m)), dim3(256), pieces_lds(h), h->stream, m, seed, nm, span[0], span[1], span[2],
                        (const RdPiece*)h->d_pieces, h->npieces, dx, dt, bs);
-    PointIO io;
-    io.xn = dx; io.out = dy; io.m = m;
-    long blocks = ((m + 63) / 64 + 3) / 4;
-    if (blocks > 2048) blocks = 2048;
-    hipLaunchKernelGGL(k_mlp_points, dim3((unsigned)blocks), dim3(256), 0, h->stream, (const RdMlpPack*)h->d_pack, io);
+    hipLaunchKernelGGL(k_mlp_points, dim3(mlp_point_blocks(m)), dim3(256), 0, h->stream, (const RdMlpPack*)h->d_pack,
+                       (const float4*)dx, m, dy);
     hipError_t e = hipGetLastError();
     std::vector<float4> t(m), y(m);
     if (e == hipSuccess) e = hipMemcpyAsync(t.data(), dt, sizeof(float4) * m, hipMemcpyDeviceToHost, h->stream);

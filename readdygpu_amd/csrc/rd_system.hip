@@ -430,6 +430,8 @@ extern "C" int rd_create(const rd_params* prm, rd_system** out)
         delete h;
         return RD_EHIP;
     }
+    if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->dev) != hipSuccess || h->ncu < 1)
+        h->ncu = 1;
     h->step = 0;
     *out = h;
     return RD_OK;
@@ -801,6 +803,7 @@ static int upload(rd_system* h)
     rc = rd_host_ext_capacity(h);
     if (rc) return rc;
     h->P.gext = h->ext_active() ? h->d_gext : nullptr;
+    h->P.ext_mask = h->comp_mask;
     h->kern = RD_KERNEL_GATHER;
     HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
     HIPCK(h, hipMemcpyAsync(h->d_reps, h->rep_dev.data(), sizeof(RdRep) * h->rep_dev.size(), hipMemcpyHostToDevice,

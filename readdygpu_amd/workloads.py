@@ -133,3 +133,50 @@ def learned_slab(n: int = 2_000_000, seed: int = 4, n_pieces: int = 512, box=(20
 
 
 LEARNED = dict(steps=500, train_samples=4_000_000)
+
+
+def crowded_cytosol(n_replicas: int = 64, seed: int = 5, nmin: int = 20_000, nmax: int = 200_000, phi: float = 0.3,
+                    n_pieces: int = 256, cell_size: float = 0.0):
+    """64 periodic cubes, particle counts log-uniform in [20,000, 200,000],
+    8 species (radii linspace(1, 5, 8) nm, D = 2 / radius nm^2/ns, assigned
+    round-robin), volume fraction 0.3; harmonic repulsion k=10 plus two
+    attractive piecewise wells (depth 1, width 1) for species pairs (0,1) and
+    (2,3); A+B (species 0,1) reaction candidates at 3 nm; dt = 0.01 ns.
+    Membrane: 256 attractive z-normal disks (k=5, radii uniform in [10, 80] nm)
+    on the plane z = 0.4 x the smallest box edge, centres uniform over the
+    largest box's x-y extent, combined by nearest piece; it acts on the two
+    largest species (6, 7).
+    Returns (spec, boxes [R,6], counts [R], xyz, types, pieces, membrane_types)."""
+    radii = np.linspace(1.0, 5.0, 8)
+    spec = SystemSpec(box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, 1.0, 1.0), dt=0.01, seed=seed, cell_size=cell_size)
+    spec.types = [ParticleType(f"S{i}", D=float(2.0 / radii[i]), radius=float(radii[i])) for i in range(8)]
+    spec.pot2 = [
+        OrderTwoPotential(HARMONIC, REPULSIVE, k=10.0),
+        OrderTwoPotential(PIECEWISE_WELL, ATTRACTIVE, k=0.0, length=1.0, depth=1.0, pairs=[(0, 1)]),
+        OrderTwoPotential(PIECEWISE_WELL, ATTRACTIVE, k=0.0, length=1.0, depth=1.0, pairs=[(2, 3)]),
+    ]
+    R = np.zeros((8, 8))
+    R[0, 1] = R[1, 0] = 3.0
+    spec.reaction_radii = R
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_replicas))).astype(np.int64)
+    vols = 4.0 / 3.0 * np.pi * radii ** 3
+    boxes = np.zeros((n_replicas, 6))
+    xyz, types = [], []
+    for r, n in enumerate(counts):
+        t = (np.arange(n) % 8).astype(np.int32)
+        L = float(np.cbrt(vols[t].sum() / phi))
+        boxes[r] = (0.0, L, 0.0, L, 0.0, L)
+        xyz.append(uniform_positions(rng, int(n), (0.0, 0.0, 0.0), (L, L, L)))
+        types.append(t)
+    lmin, lmax = float(boxes[:, 1].min()), float(boxes[:, 1].max())
+    zm = 0.4 * lmin
+    pieces = []
+    for i in range(n_pieces):
+        c = (float(rng.uniform(0, lmax)), float(rng.uniform(0, lmax)), zm)
+        pieces.append(OrderOnePotential(DISK, ATTRACTIVE, k=5.0, origin=c, normal=(0.0, 0.0, 1.0),
+                                        radius=float(rng.uniform(10.0, 80.0))))
+    return spec, boxes, counts, np.concatenate(xyz), np.concatenate(types), pieces, [6, 7]
+
+
+CYTOSOL = dict(steps=1000, rdf_range=20.0, rdf_bins=100)

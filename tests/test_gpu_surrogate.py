@@ -110,13 +110,14 @@ def test_training_reduces_the_force_error():
     assert r1 < 0.5 * r0
 
 
-def test_surrogate_in_the_step():
-    """The step with the surrogate adds gscale * mlp(xn) to the pair gradient,
-    and runs stably."""
+@pytest.mark.parametrize("mtypes", [None, [1]])
+def test_surrogate_in_the_step(mtypes):
+    """The step with the surrogate adds gscale * mlp(xn) to the pair gradient
+    of the particles whose type the composite acts on, and runs stably."""
     spec, pieces, xyz, types = small_system(n_pieces=4)
     o = orc.OracleSystem(spec)
     with Simulation(spec, xyz, types) as sim:
-        sim.set_composite(pieces)
+        sim.set_composite(pieces, types=mtypes)
         sim.surrogate_init(seed=2)
         sim.surrogate_make_data(1 << 16, seed=3)
         sim.surrogate_train(steps=200, batch=4096)
@@ -130,6 +131,8 @@ def test_surrogate_in_the_step():
         x50, _ = sim.copy_pos_from_device()
     gp, _ = o.gradients(xyz.astype(np.float64), types, brute=True)
     ext = y.astype(np.float64) * np.float32(gsc)
+    if mtypes is not None:
+        ext = np.where(np.isin(types, mtypes)[:, None], ext, 0.0)
     tol = 2e-5 * np.abs(gp).max() + 1e-5 + 1e-5 * np.abs(ext).max()
     assert np.abs(g - (gp + ext)).max() <= tol
     assert np.isfinite(x50).all()
