@@ -20,6 +20,7 @@ extern "C" {
 
 #define RD_MAX_TYPES 16        /* particle types per system */
 #define RD_MAX_POT2 8          /* order-two potentials per system */
+#define RD_MAX_BOND_POTS 8     /* group (bond) potentials per system */
 #define RD_MAX_PARTICLES (1 << 26)
 
 enum {
@@ -112,6 +113,18 @@ int rd_add_order_two_potential(rd_system* h, const rd_pot2* pot, const uint8_t* 
 /* reaction radius per type pair (0: no reaction); candidates are pairs with
  * dist < R strictly (ReactionObserver.java:101-130) */
 int rd_set_reaction_radii(rd_system* h, const double* radius_matrix);
+/* Bonded groups: the groups kernel (CudaSimulation.cu:1829-1836) over the
+ * GroupPart list and group potential table built in CudaSimulation.cu:394-435
+ * from the group templates (Simulation.cpp:254-264, GroupParametersHandler.cpp)
+ * and instances (GroupReader.cpp:13-40).  n_bonds pairs of global particle
+ * indices bonds[2b], bonds[2b+1]; bond b uses pots[pot_index[b]] (HARMONIC,
+ * WEAK_HARMONIC or PIECEWISE_WELL) with rest length r0[p] (NULL or < 0: the
+ * collision radius sum of the two particles' types, as P2_Harmonic.java:122-126),
+ * minimum-image distance and no cutoff.  Both particles of a bond must lie in
+ * the same replica.  Reactions renumber bonds with their particles and drop
+ * those of removed particles.  n_bonds = 0 removes every bond. */
+int rd_set_bonds(rd_system* h, int n_bonds, const int* bonds, const int* pot_index, int n_pots, const rd_pot2* pots,
+                 const double* r0);
 /* CoordReader (CoordReader.cpp:16-65) + copyPosToDevice/copyTypesToDevice
  * (CudaSimulation.cu:905-970).  Particle index = position in these arrays. */
 int rd_set_particles(rd_system* h, int n, const float* xyz, const int* type);
@@ -225,13 +238,16 @@ int rd_surrogate_validate(rd_system* h, long m, uint64_t seed, double* rel_rmse)
  * parse (Simulation.cpp:206-338): param_global (GlobalParametersHandler.cpp:
  * 73-143), param_particles (ParticleParametersHandler.cpp:48-162),
  * tplgy_potentials (PotentialParametersHandler.cpp:38-83), tplgy_coordinates
- * (CoordReader.cpp:16-65).  Parsing needs no GPU. */
+ * (CoordReader.cpp:16-65), param_groups (GroupParametersHandler.cpp) and
+ * tplgy_groups (GroupReader.cpp:13-40).  Parsing needs no GPU. */
 typedef struct rd_model rd_model;
 typedef struct {
     int n_particles;
     int n_types;
     int n_pot1, n_pot2;
-    int n_groups;            /* group instances found (bonded groups are not simulated) */
+    int n_groups;            /* group instances (tplgy_groups <g>) */
+    int n_bonds;             /* bonded pairs they define */
+    int n_bond_pots;         /* group potentials of all templates, in file order */
     int frames;              /* nSimulationSteps */
     int steps_per_frame;     /* round(dt / dtOpenMM) (XMLReader.cpp:67) */
     double dt_frame;         /* dt */
@@ -252,6 +268,10 @@ int rd_model_types(const rd_model* m, double* D, double* radius, double* radii_m
 int rd_model_particles(const rd_model* m, float* xyz, int* type, int* readdy_id);
 int rd_model_pot1(const rd_model* m, int i, rd_pot1* pot);
 int rd_model_pot2(const rd_model* m, int i, rd_pot2* pot, uint8_t* pair_mask);
+/* bonds [2*n_bonds] (particle indices in file order), pot_index [n_bonds],
+ * pots [n_bond_pots], r0 [n_bond_pots] (< 0: collision radius sum); any
+ * output may be NULL */
+int rd_model_bonds(const rd_model* m, int* bonds, int* pot_index, rd_pot2* pots, double* r0);
 /* Simulation + CudaSimulation::initialize from a parsed model (maxCutoff as
  * given; the reference JNI path hard-codes 10, lib/ReaDDyGPU.cpp:101) */
 int rd_create_from_model(const rd_model* m, double max_cutoff, uint64_t seed, rd_system** out);

@@ -35,7 +35,9 @@ class orc_system(C.Structure):
     _fields_ = [("n_types", C.c_int), ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("cell", C.c_double),
                 ("kbt", C.c_double), ("dt", C.c_double), ("D", C.c_void_p), ("prad", C.c_void_p),
                 ("r0", C.c_void_p), ("rc2", C.c_void_p), ("n_pot1", C.c_int), ("pot1", C.c_void_p),
-                ("n_pot2", C.c_int), ("pot2", C.c_void_p), ("seed", C.c_uint64), ("stream", C.c_uint32)]
+                ("n_pot2", C.c_int), ("pot2", C.c_void_p), ("seed", C.c_uint64), ("stream", C.c_uint32),
+                ("n_bonds", C.c_int), ("bonds", C.c_void_p), ("bond_pot", C.c_void_p), ("bpot", C.c_void_p),
+                ("bond_r0", C.c_void_p)]
 
 
 def _p(a):
@@ -140,6 +142,21 @@ class OracleSystem:
         s.n_pot1, s.pot1 = len(spec.pot1), C.cast(p1, C.c_void_p)
         s.n_pot2, s.pot2 = len(spec.pot2), C.cast(p2, C.c_void_p)
         s.seed, s.stream = spec.seed, spec.stream
+        s.n_bonds = 0
+        if spec.bonds is not None and len(spec.bonds) > 0:
+            self._bonds = np.ascontiguousarray(spec.bonds, dtype=np.int32).reshape(-1, 2)
+            nb = len(self._bonds)
+            self._bond_pot = (np.zeros(nb, dtype=np.int32) if spec.bond_pot is None
+                              else np.ascontiguousarray(spec.bond_pot, dtype=np.int32))
+            bp = (orc_pot2 * len(spec.bond_pots))()
+            for i, q in enumerate(spec.bond_pots):
+                bp[i] = make_pot2(q.kind, q.subtype, q.k, q.length, q.depth)
+            self._bp = bp
+            r0 = [-1.0] * len(spec.bond_pots) if spec.bond_r0 is None else \
+                [-1.0 if v is None else float(v) for v in spec.bond_r0]
+            self._bond_r0 = np.array(r0, dtype=np.float64)
+            s.n_bonds, s.bonds, s.bond_pot = nb, _p(self._bonds), _p(self._bond_pot)
+            s.bpot, s.bond_r0 = C.cast(bp, C.c_void_p), _p(self._bond_r0)
         self.s = s
 
     @property

@@ -372,6 +372,33 @@ static double pair_all(const orc_system* s, int ti, int tj, const double* d, dou
     return e;
 }
 
+/* Bonded pairs (CudaSimulation.cu:1829-1836 with each group template's own
+ * potential and the pair's radius sum, P2_Harmonic.java:122-126): the
+ * minimum-image separation, no cutoff; gradient added to both particles,
+ * energy counted once.  Returns the bond energy. */
+static double bonds_all(const orc_system* s, const double* xyz, const int* type, double* grad)
+{
+    double Lb[3], e = 0.0;
+    for (int d = 0; d < 3; ++d) Lb[d] = s->hi[d] - s->lo[d];
+    int nt = s->n_types;
+    for (int b = 0; b < s->n_bonds; ++b) {
+        int i = s->bonds[2 * b], j = s->bonds[2 * b + 1];
+        const orc_pot2* p = &s->bpot[s->bond_pot[b]];
+        orc_pot2 q = *p;
+        q.cutoff = INFINITY;
+        double r0 = s->bond_r0[s->bond_pot[b]];
+        if (r0 < 0) r0 = s->r0[type[i] * nt + type[j]];
+        double d[3], g[3];
+        for (int k = 0; k < 3; ++k) d[k] = mi64(xyz[3 * i + k] - xyz[3 * j + k], Lb[k]);
+        e += orc_pair_eval(&q, r0, d, g);
+        for (int k = 0; k < 3; ++k) {
+            grad[3 * i + k] += g[k];
+            grad[3 * j + k] -= g[k];
+        }
+    }
+    return e;
+}
+
 double orc_gradients_bruteforce(const orc_system* s, int n, const double* xyz, const int* type,
                                 double* grad)
 {
@@ -390,7 +417,7 @@ double orc_gradients_bruteforce(const orc_system* s, int n, const double* xyz, c
         }
         grad[3 * i] = g[0]; grad[3 * i + 1] = g[1]; grad[3 * i + 2] = g[2];
     }
-    return etot;
+    return etot + bonds_all(s, xyz, type, grad);
 }
 
 /* ---- cell list shared by the cell-based routines ---- */
@@ -507,7 +534,7 @@ double orc_gradients_cells(const orc_system* s, int n, const double* xyz, const 
     }
     clist_free(&cl);
     free(xf);
-    return etot;
+    return etot + bonds_all(s, xyz, type, grad);
 }
 
 /* ======================================================================

@@ -14,6 +14,8 @@
  *   Euler-Maruyama update        CudaSimulation.cu:1838-1919 (x += -dt*D*dU/kT + sqrt(2Ddt)*N)
  *   RDF histogram                CudaSimulation.cu:1784-1827
  *   reaction candidates          ReactionObserver.java:101-130 (dist < R strictly)
+ *   bonded groups                CudaSimulation.cu:1829-1836 (groups kernel), pairs and
+ *                                potentials from the group templates (Simulation.cpp:254-264)
  *
  * Trajectories and gradients are FP64.  Set-valued outputs (cell ids, pair
  * sets, RDF bins) are evaluated with the canonical FP32 expressions spelled
@@ -75,6 +77,11 @@ typedef struct {
     const orc_pot2* pot2;
     uint64_t seed;
     uint32_t stream;           /* replica id for batched runs */
+    int n_bonds;               /* bonded pairs of group potentials */
+    const int* bonds;          /* [2*n_bonds] particle indices */
+    const int* bond_pot;       /* [n_bonds] index into bpot */
+    const orc_pot2* bpot;      /* bond potentials (mask unused, cutoff ignored) */
+    const double* bond_r0;     /* per bond potential: rest length, < 0: r0 of the type pair */
 } orc_system;
 
 /* ---- noise ---- */

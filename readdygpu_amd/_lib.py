@@ -66,6 +66,8 @@ class rd_model_info(C.Structure):
         ("n_pot1", C.c_int),
         ("n_pot2", C.c_int),
         ("n_groups", C.c_int),
+        ("n_bonds", C.c_int),
+        ("n_bond_pots", C.c_int),
         ("frames", C.c_int),
         ("steps_per_frame", C.c_int),
         ("dt_frame", C.c_double),
@@ -114,6 +116,7 @@ _SIGS = {
     "rd_add_order_one_potential": (C.c_int, [P, C.POINTER(rd_pot1)]),
     "rd_add_order_two_potential": (C.c_int, [P, C.POINTER(rd_pot2), P]),
     "rd_set_reaction_radii": (C.c_int, [P, P]),
+    "rd_set_bonds": (C.c_int, [P, C.c_int, P, P, C.c_int, C.POINTER(rd_pot2), P]),
     "rd_set_particles": (C.c_int, [P, C.c_int, P, P]),
     "rd_set_replicas": (C.c_int, [P, C.c_int, P, P, P, P, P]),
     "rd_initialize": (C.c_int, [P]),
@@ -142,6 +145,7 @@ _SIGS = {
     "rd_model_particles": (C.c_int, [P, P, P, P]),
     "rd_model_pot1": (C.c_int, [P, C.c_int, C.POINTER(rd_pot1)]),
     "rd_model_pot2": (C.c_int, [P, C.c_int, C.POINTER(rd_pot2), P]),
+    "rd_model_bonds": (C.c_int, [P, P, P, P, P]),
     "rd_create_from_model": (C.c_int, [P, C.c_double, C.c_uint64, C.POINTER(P)]),
     "rd_set_composite": (C.c_int, [P, C.c_int, P, C.c_uint32]),
     "rd_set_external_mode": (C.c_int, [P, C.c_int]),

@@ -9,6 +9,7 @@
 
 #define RD_MAXT 16
 #define RD_MAXP2 8
+#define RD_MAXBP 8      // bond (group) potentials
 
 // order-two potential, device form (CudaOrderTwoPotential, CudaSimulation.cuh:30-37)
 struct RdPot2Dev {
@@ -65,6 +66,15 @@ struct RdDev {
     RdPot2Dev pot2[RD_MAXP2];
     const float4* gext;   // external gradient per slot (x,y,z, energy) from a separate pass, or null
     uint32_t ext_mask;    // types that gext applies to (other slots of gext are not written)
+    // bonded pairs of group potentials, listed per particle (both directions)
+    // by global index; bstart is null when there are none
+    int n_bpot;
+    RdPot2Dev bpot[RD_MAXBP];
+    float bpot_r0[RD_MAXBP];      // fixed rest length, or < 0: collision radius sum of the pair
+    const int* bstart;            // [n+1]
+    const int2* bonds;            // (partner global index, potential)
+    const uint32_t* slot_of;      // global index -> slot of the current sorted array
+    uint32_t* gid_n;              // global index of each slot of the next-step array
 };
 
 // packed particle word: low 6 bits type, high 26 bits particle index

@@ -82,6 +82,16 @@ struct rd_system {
     int noise_n = -1;             // particle count the noise array was laid out for
     size_t io_cap = 0;
 
+    // bonded pairs of group potentials (global particle indices)
+    std::vector<int> bond_pairs;      // 2 per bond
+    std::vector<int> bond_pot;        // potential per bond
+    std::vector<rd_pot2> bpots;
+    std::vector<double> bpot_r0;      // < 0: collision radius sum
+    int* d_bstart = nullptr;
+    int2* d_bonds = nullptr;
+    uint32_t* d_slot = nullptr;       // global index -> slot (null without bonds)
+    uint32_t* d_gid = nullptr;        // slot of the next-step array -> global index (null without bonds)
+
     // RDF
     unsigned long long* d_rdf = nullptr;
     double rdf_rmax = 0.0;

@@ -248,6 +248,36 @@ __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const
     return make_float3(gx, gy, gz);
 }
 
+// bonded partners of the particle in slot s (group potentials,
+// CudaSimulation.cu:1829-1836): minimum-image distance, no cutoff, added to
+// the fixed-point pair sum
+template <bool ENERGY>
+__device__ __forceinline__ void bond_accum(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
+                                           const float4* __restrict__ ps, float4 p, long long& ax, long long& ay,
+                                           long long& az, float& e)
+{
+    const uint32_t gi = (uint32_t)R.off + w_index(p.w);
+    const int b1 = P->bstart[gi + 1];
+    const int rowbase = w_type(p.w) * P->nt;
+    for (int b = P->bstart[gi]; b < b1; ++b) {
+        const int2 bd = P->bonds[b];
+        const float4 q = ps[P->slot_of[bd.x]];
+        const float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
+        const float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
+        const float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
+        const float d2 = dist2(dx, dy, dz);
+        if (!(d2 > 0.0f)) continue;
+        const float rinv = __builtin_amdgcn_rsqf(d2);
+        const float r = d2 * rinv;
+        const float fr0 = P->bpot_r0[bd.y];
+        const float r0 = fr0 >= 0.0f ? fr0 : T.r0[rowbase + w_type(q.w)];
+        float ep = 0.0f;
+        const float pre = pair_pre_fast<ENERGY>(P->bpot[bd.y], r0, r, rinv, &ep);
+        ax += to_fix(pre * dx); ay += to_fix(pre * dy); az += to_fix(pre * dz);
+        if (ENERGY) e += 0.5f * ep;
+    }
+}
+
 // per-wave list of candidate pairs that passed the distance test (slot of
 // the partner), transposed so that a wave's lanes write one LDS row
 #define RD_HITS 16
@@ -343,6 +373,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
             });
         }
     }
+    if (P->bstart) bond_accum<ENERGY>(P, R, T, ps, p, ax, ay, az, e);
     float3 g = finish_grad<ENERGY>(P, T, pot1, s, p, ax, ay, az, &e);
     if (ENERGY) *energy = e;
     return g;
@@ -390,6 +421,7 @@ __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, con
     zc = wrap_coord(zc, R.lo[2], R.hi[2], R.Lb[2]);
     if (!(fabsf(x) < 3.0e38f && fabsf(y) < 3.0e38f && fabsf(zc) < 3.0e38f)) atomicOr(&st->err, 1u);
     pn[s] = make_float4(x, y, zc, p.w);
+    if (P->gid_n) P->gid_n[s] = (uint32_t)R.off + idx;
     uint32_t k = cell_key(R, x, y, zc);
     key[s] = k;
     rank[s] = atomicAdd(&count[k], 1u);

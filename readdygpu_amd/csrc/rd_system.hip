@@ -23,13 +23,14 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pack(const float* __restrict__ xyz
                                                    const int* __restrict__ rep_of, int n,
                                                    const RdRep* __restrict__ reps, float4* __restrict__ pn,
                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
-                                                   uint32_t* __restrict__ count)
+                                                   uint32_t* __restrict__ count, uint32_t* __restrict__ gid_n)
 {
     int i = blockIdx.x * RD_BLOCK + threadIdx.x;
     if (i >= n) return;
     const RdRep R = reps[rep_of[i]];
     float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
     pn[i] = make_float4(x, y, z, w_pack((uint32_t)(i - R.off), type[i]));
+    if (gid_n) gid_n[i] = (uint32_t)i;
     uint32_t k = cell_key(R, x, y, z);
     key[i] = k;
     rank[i] = atomicAdd(&count[k], 1u);
@@ -220,13 +221,15 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ 
 __global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__ pn, const uint32_t* __restrict__ key,
                                                       const uint32_t* __restrict__ rank,
                                                       const uint32_t* __restrict__ start, float4* __restrict__ ps,
-                                                      int n, RdStepState* __restrict__ st, int advance)
+                                                      int n, RdStepState* __restrict__ st, int advance,
+                                                      const uint32_t* __restrict__ gid_n, uint32_t* __restrict__ slot_of)
 {
     int s = blockIdx.x * RD_BLOCK + threadIdx.x;
     if (advance && s == 0) st->step += 1ull;
     if (s >= n) return;
     uint32_t dst = start[key[s]] + rank[s];
     ps[dst] = pn[s];
+    if (slot_of) slot_of[gid_n[s]] = dst;
 }
 
 // positions back to particle-index order
@@ -397,8 +400,17 @@ static void drop_graph(rd_system* h)
     h->graphs.clear();
 }
 
+static void free_bonds(rd_system* h)
+{
+    void* ptrs[] = { h->d_bstart, h->d_bonds, h->d_slot, h->d_gid };
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_bstart = nullptr; h->d_bonds = nullptr; h->d_slot = nullptr; h->d_gid = nullptr;
+}
+
 static void free_dev(rd_system* h)
 {
+    free_bonds(h);
     void* ptrs[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_bsum,
                      h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_pairs, h->d_pcount,
                      h->d_xyz_io, h->d_type_io, h->d_etot, h->d_reps, h->d_waves, h->d_rep_of };
@@ -505,6 +517,33 @@ extern "C" int rd_set_reaction_radii(rd_system* h, const double* R)
             double v = std::max(h->rxn[i * h->nt + j], h->rxn[j * h->nt + i]);
             h->rxn[i * h->nt + j] = h->rxn[j * h->nt + i] = v;
         }
+    h->dirty = true;
+    return RD_OK;
+}
+
+extern "C" int rd_set_bonds(rd_system* h, int n_bonds, const int* bonds, const int* pot_index, int n_pots,
+                            const rd_pot2* pots, const double* r0)
+{
+    if (!h || n_bonds < 0 || n_pots < 0) return RD_EINVAL;
+    if (n_bonds > 0 && (!bonds || !pot_index || n_pots == 0 || !pots)) return RD_EINVAL;
+    static_assert(RD_MAXBP == RD_MAX_BOND_POTS, "bond potential limits differ");
+    if (n_pots > RD_MAXBP) return set_err(h, RD_EINVAL, "too many bond potentials (at most " + std::to_string(RD_MAXBP) + ")");
+    for (int p = 0; p < n_pots; ++p) {
+        const int t = pots[p].type;
+        if (t != RD_POT2_HARMONIC && t != RD_POT2_WEAK_HARMONIC && t != RD_POT2_PIECEWISE_WELL)
+            return set_err(h, RD_EINVAL, "unknown bond potential type");
+    }
+    for (int b = 0; b < n_bonds; ++b) {
+        const int a = bonds[2 * b], c = bonds[2 * b + 1];
+        if (a < 0 || c < 0 || a == c) return set_err(h, RD_EINVAL, "a bond needs two distinct particle indices");
+        if (pot_index[b] < 0 || pot_index[b] >= n_pots) return set_err(h, RD_EINVAL, "bond potential index out of range");
+    }
+    h->bond_pairs.assign(bonds, bonds + 2 * (size_t)n_bonds);
+    h->bond_pot.assign(pot_index, pot_index + n_bonds);
+    h->bpots.assign(pots, pots + n_pots);
+    h->bpot_r0.assign((size_t)n_pots, -1.0);
+    if (r0)
+        for (int p = 0; p < n_pots; ++p) h->bpot_r0[p] = r0[p];
     h->dirty = true;
     return RD_OK;
 }
@@ -773,7 +812,8 @@ static void launch_sort(rd_system* h, int advance)
     hipLaunchKernelGGL(k_scan_apply, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum,
                        h->d_start);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(h->n, 1), RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_pn,
-                       h->d_key, h->d_rank, h->d_start, h->d_ps, h->n, h->d_st, advance);
+                       h->d_key, h->d_rank, h->d_start, h->d_ps, h->n, h->d_st, advance,
+                       (const uint32_t*)h->d_gid, h->d_slot);
 }
 
 static int wave_blocks(const rd_system* h) { return std::max(1, nblk(h->nwaves, RD_BLOCK / 64)); }
@@ -793,6 +833,57 @@ static void launch_step(rd_system* h)
     launch_sort(h, 1);
 }
 
+// bonds as per-particle lists (both directions) by global index, the bond
+// potential table, and the slot map the force kernel reads partners through
+static int upload_bonds(rd_system* h)
+{
+    RdDev& P = h->P;
+    free_bonds(h);
+    P.bstart = nullptr; P.bonds = nullptr; P.slot_of = nullptr; P.gid_n = nullptr;
+    P.n_bpot = (int)h->bpots.size();
+    for (int p = 0; p < P.n_bpot; ++p) {
+        const rd_pot2& q = h->bpots[p];
+        RdPot2Dev& d = P.bpot[p];
+        memset(&d, 0, sizeof(d));
+        d.kind = q.type; d.subtype = q.subtype; d.k = q.force_const;
+        d.length = q.length; d.depth = q.depth; d.cutoff = INFINITY;
+        d.wc = q.type == RD_POT2_PIECEWISE_WELL ? (float)(q.depth / (0.25 * q.length * q.length)) : 0.0f;
+        P.bpot_r0[p] = (float)h->bpot_r0[p];
+    }
+    const size_t nb = h->bond_pot.size();
+    if (nb == 0 || h->n <= 0) return RD_OK;
+    // replica of a global index: replicas hold consecutive index ranges
+    auto rep_of = [&](int i) {
+        size_t r = 0;
+        while (r + 1 < h->rep_dev.size() && h->rep_dev[r + 1].off <= i) ++r;
+        return r;
+    };
+    std::vector<int> start((size_t)h->n + 1, 0);
+    for (size_t b = 0; b < nb; ++b) {
+        const int a = h->bond_pairs[2 * b], c = h->bond_pairs[2 * b + 1];
+        if (a >= h->n || c >= h->n) return set_err(h, RD_EINVAL, "a bond names a particle index beyond the particle count");
+        if (h->rep_dev.size() > 1 && rep_of(a) != rep_of(c)) return set_err(h, RD_EINVAL, "a bond joins two replicas");
+        start[a + 1]++; start[c + 1]++;
+    }
+    for (int i = 0; i < h->n; ++i) start[i + 1] += start[i];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    std::vector<int2> ent(2 * nb);
+    for (size_t b = 0; b < nb; ++b) {
+        const int a = h->bond_pairs[2 * b], c = h->bond_pairs[2 * b + 1], p = h->bond_pot[b];
+        ent[fill[a]++] = make_int2(c, p);
+        ent[fill[c]++] = make_int2(a, p);
+    }
+    HIPCK(h, hipMalloc(&h->d_bstart, sizeof(int) * start.size()));
+    HIPCK(h, hipMalloc(&h->d_bonds, sizeof(int2) * ent.size()));
+    HIPCK(h, hipMalloc(&h->d_slot, sizeof(uint32_t) * (size_t)h->n));
+    HIPCK(h, hipMalloc(&h->d_gid, sizeof(uint32_t) * (size_t)h->n));
+    HIPCK(h, hipMemcpyAsync(h->d_bstart, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->d_bonds, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    P.bstart = h->d_bstart; P.bonds = h->d_bonds; P.slot_of = h->d_slot; P.gid_n = h->d_gid;
+    return RD_OK;
+}
+
 // upload tables and particles, build the cell-sorted arrays
 static int upload(rd_system* h)
 {
@@ -801,6 +892,8 @@ static int upload(rd_system* h)
     rc = ensure_capacity(h);
     if (rc) return rc;
     rc = rd_host_ext_capacity(h);
+    if (rc) return rc;
+    rc = upload_bonds(h);
     if (rc) return rc;
     h->P.gext = h->ext_active() ? h->d_gext : nullptr;
     h->P.ext_mask = h->comp_mask;
@@ -834,7 +927,7 @@ static int upload(rd_system* h)
         HIPCK(h, hipMemcpyAsync(h->d_type_io, h->htype.data(), sizeof(int) * h->n, hipMemcpyHostToDevice, h->stream));
         HIPCK(h, hipMemcpyAsync(h->d_rep_of, h->hrep_of.data(), sizeof(int) * h->n, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io, h->d_type_io,
-                           h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count);
+                           h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_gid);
     }
     launch_sort(h, 0);
     HIPCK(h, hipGetLastError());
@@ -1260,11 +1353,25 @@ extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
     }
     std::vector<float> cx;
     std::vector<int> ct;
+    std::vector<int> remap(ntp.size(), -1);
     for (size_t i = 0; i < ntp.size(); ++i)
         if (ntp[i] != -1) {
+            remap[i] = (int)ct.size();
             cx.push_back(nx[3 * i]); cx.push_back(nx[3 * i + 1]); cx.push_back(nx[3 * i + 2]);
             ct.push_back(ntp[i]);
         }
+    // bonds follow their particles; a bond of a removed particle is dropped
+    if (!h->bond_pot.empty()) {
+        std::vector<int> bp, bo;
+        auto rm = [&](int i) { return i < (int)remap.size() ? remap[i] : -1; };
+        for (size_t b = 0; b < h->bond_pot.size(); ++b) {
+            const int a = rm(h->bond_pairs[2 * b]), c = rm(h->bond_pairs[2 * b + 1]);
+            if (a < 0 || c < 0) continue;
+            bp.push_back(a); bp.push_back(c); bo.push_back(h->bond_pot[b]);
+        }
+        h->bond_pairs.swap(bp);
+        h->bond_pot.swap(bo);
+    }
     rc = rd_set_particles(h, (int)ct.size(), cx.data(), ct.data());
     if (rc) return rc;
     return refresh(h);

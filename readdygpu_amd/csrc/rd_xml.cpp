@@ -161,6 +161,9 @@ struct rd_model {
     std::vector<rd_pot1> pot1;
     std::vector<rd_pot2> pot2;
     std::vector<std::vector<uint8_t>> pot2mask;
+    std::vector<int> bonds, bond_pot;    // bonded pairs of the group instances
+    std::vector<rd_pot2> bpots;          // group potentials, templates in file order
+    std::vector<double> bpot_r0;
 };
 
 static double numf(const XNode* n, double def)
@@ -353,14 +356,91 @@ extern "C" int rd_model_read_xml(const char* param_particles, const char* tplgy_
         m->info.n_particles = (int)m->type.size();
     }
 
-    // --- groups: instances in tplgy_groups are counted only (bonded group
-    //     forces, CudaSimulation.cu:1829-1836, are not part of this step);
-    //     param_groups (group templates) is not read ---
-    (void)param_groups;
+    // --- param_groups (GroupParametersHandler.cpp, Simulation.cpp:254-264):
+    //     one template per <particleGroup>, its <pot> entries with
+    //     affectedInternalIdPairs; tplgy_groups (GroupReader.cpp:13-40):
+    //     <g type="template" internalAndParticleId="[[internal,particleId];...]"/>.
+    //     Every affected internal pair of every instance becomes a bond with
+    //     that template's potential (the reference GPU code bonds only the
+    //     first two members, always with potential 0).  Missing subtype:
+    //     spring, missing forceConst: 10 (P2_Harmonic.java:47-49 defaults). ---
+    struct Tmpl {
+        int first = 0;                                  // index of its first potential in bpots
+        std::vector<std::vector<std::pair<int, int>>> pairs;   // per potential: internal id pairs
+    };
+    std::vector<Tmpl> tmpls;
+    // a missing group file means no groups, as GroupReader.cpp:17-19
+    if (param_groups && param_groups[0] && read_file(param_groups, txt)) {
+        if (!xml_parse(txt, doc, e)) { delete m; return fail(err, err_len, "param_groups: " + e); }
+        std::vector<const XNode*> gs;
+        collect(doc, "particleGroup", gs);
+        for (const XNode* g : gs) {
+            Tmpl T;
+            T.first = (int)m->bpots.size();
+            std::vector<const XNode*> pots;
+            collect(*g, "pot", pots);
+            for (const XNode* p : pots) {
+                auto A = [&](const char* k) -> std::string {
+                    auto it = p->attrs.find(k);
+                    return it == p->attrs.end() ? std::string() : trim(it->second);
+                };
+                std::string type = A("type");
+                rd_pot2 q;
+                memset(&q, 0, sizeof(q));
+                if (type == "HARMONIC") q.type = RD_POT2_HARMONIC;
+                else if (type == "WEAK_INTERACTION_HARMONIC") q.type = RD_POT2_WEAK_HARMONIC;
+                else if (type == "WEAK_INTERACTION_PIECEWISE_HARMONIC") q.type = RD_POT2_PIECEWISE_WELL;
+                else continue;                              // not an order-two form (as Simulation.cpp:257-261)
+                std::string sub = A("subtype");
+                q.subtype = sub.empty() ? RD_SUB_SPRING : subtype_id(sub);
+                std::string k = A("forceConst");
+                q.force_const = k.empty() ? 10.0f : (float)atof(k.c_str());
+                q.length = (float)atof(A("length").c_str());
+                q.depth = (float)atof(A("depth").c_str());
+                std::string r0 = A("r0");
+                std::vector<double> v = numbers(A("affectedInternalIdPairs"));
+                std::vector<std::pair<int, int>> pr;
+                for (size_t t = 0; t + 1 < v.size(); t += 2) pr.emplace_back((int)v[t], (int)v[t + 1]);
+                if ((int)m->bpots.size() >= RD_MAX_BOND_POTS) { delete m; return fail(err, err_len, "param_groups: too many group potentials"); }
+                m->bpots.push_back(q);
+                m->bpot_r0.push_back(r0.empty() || r0 == "null" ? -1.0 : atof(r0.c_str()));
+                T.pairs.push_back(pr);
+            }
+            tmpls.push_back(T);
+        }
+    }
     int ngroups = 0;
-    if (tplgy_groups && read_file(tplgy_groups, txt) && xml_parse(txt, doc, e))
-        for (const XNode& top : doc.kids) ngroups += (int)top.kids.size();
+    if (tplgy_groups && tplgy_groups[0] && read_file(tplgy_groups, txt)) {
+        if (!xml_parse(txt, doc, e)) { delete m; return fail(err, err_len, "tplgy_groups: " + e); }
+        std::map<int, int> index_of;                   // ReaDDy particle id -> index
+        for (size_t i = 0; i < m->rid.size(); ++i) index_of[m->rid[i]] = (int)i;
+        std::vector<const XNode*> gs;
+        collect(doc, "g", gs);
+        for (const XNode* g : gs) {
+            ++ngroups;
+            int t = atoi(g->attrs.count("type") ? g->attrs.at("type").c_str() : "-1");
+            if (t < 0 || t >= (int)tmpls.size()) { delete m; return fail(err, err_len, "tplgy_groups: unknown group type " + std::to_string(t)); }
+            std::vector<double> v = numbers(g->attrs.count("internalAndParticleId") ? g->attrs.at("internalAndParticleId") : "");
+            std::map<int, int> member;                 // internal id -> particle index
+            for (size_t k = 0; k + 1 < v.size(); k += 2) {
+                auto it = index_of.find((int)v[k + 1]);
+                if (it == index_of.end()) { delete m; return fail(err, err_len, "tplgy_groups: unknown particle id " + std::to_string((int)v[k + 1])); }
+                member[(int)v[k]] = it->second;
+            }
+            const Tmpl& T = tmpls[t];
+            for (size_t p = 0; p < T.pairs.size(); ++p)
+                for (const auto& pr : T.pairs[p]) {
+                    auto a = member.find(pr.first), c = member.find(pr.second);
+                    if (a == member.end() || c == member.end()) { delete m; return fail(err, err_len, "tplgy_groups: a group lacks a member its potentials name"); }
+                    m->bonds.push_back(a->second);
+                    m->bonds.push_back(c->second);
+                    m->bond_pot.push_back(T.first + (int)p);
+                }
+        }
+    }
     m->info.n_groups = ngroups;
+    m->info.n_bonds = (int)m->bond_pot.size();
+    m->info.n_bond_pots = (int)m->bpots.size();
     *out = m;
     return RD_OK;
 }
@@ -409,6 +489,16 @@ extern "C" int rd_model_pot2(const rd_model* m, int i, rd_pot2* p, uint8_t* mask
     return RD_OK;
 }
 
+extern "C" int rd_model_bonds(const rd_model* m, int* bonds, int* pot_index, rd_pot2* pots, double* r0)
+{
+    if (!m) return RD_EINVAL;
+    if (bonds && !m->bonds.empty()) memcpy(bonds, m->bonds.data(), sizeof(int) * m->bonds.size());
+    if (pot_index && !m->bond_pot.empty()) memcpy(pot_index, m->bond_pot.data(), sizeof(int) * m->bond_pot.size());
+    if (pots && !m->bpots.empty()) memcpy(pots, m->bpots.data(), sizeof(rd_pot2) * m->bpots.size());
+    if (r0 && !m->bpot_r0.empty()) memcpy(r0, m->bpot_r0.data(), sizeof(double) * m->bpot_r0.size());
+    return RD_OK;
+}
+
 extern "C" int rd_create_from_model(const rd_model* m, double max_cutoff, uint64_t seed, rd_system** out)
 {
     if (!m || !out) return RD_EINVAL;
@@ -433,6 +523,9 @@ extern "C" int rd_create_from_model(const rd_model* m, double max_cutoff, uint64
     for (size_t k = 0; rc == RD_OK && k < m->pot2.size(); ++k)
         rc = rd_add_order_two_potential(h, &m->pot2[k], m->pot2mask[k].data());
     if (rc == RD_OK) rc = rd_set_particles(h, I.n_particles, m->xyz.data(), m->type.data());
+    if (rc == RD_OK && !m->bond_pot.empty())
+        rc = rd_set_bonds(h, (int)m->bond_pot.size(), m->bonds.data(), m->bond_pot.data(), (int)m->bpots.size(),
+                          m->bpots.data(), m->bpot_r0.data());
     if (rc == RD_OK) rc = rd_initialize(h);
     if (rc) {
         fprintf(stderr, "rd_create_from_model: %s\n", rd_last_error(h));

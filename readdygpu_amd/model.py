@@ -74,6 +74,13 @@ class SystemSpec:
     pot1: List[OrderOnePotential] = field(default_factory=list)
     pot2: List[OrderTwoPotential] = field(default_factory=list)
     reaction_radii: Optional[np.ndarray] = None   # [nt, nt], 0 = no reaction
+    # bonded groups (group potentials): pairs of global particle indices, the
+    # potential of each bond, and per potential a rest length (None or < 0:
+    # the collision radius sum of the pair's types)
+    bonds: Optional[np.ndarray] = None            # [nb, 2]
+    bond_pot: Optional[np.ndarray] = None         # [nb] index into bond_pots (None: all 0)
+    bond_pots: List[OrderTwoPotential] = field(default_factory=list)
+    bond_r0: Optional[Sequence[float]] = None
 
     @property
     def n_types(self) -> int:

@@ -82,6 +82,8 @@ class Simulation:
         if spec.reaction_radii is not None:
             R = np.ascontiguousarray(spec.reaction_radii, dtype=np.float64)
             self._check(self.lib.rd_set_reaction_radii(h, _ptr(R)))
+        if spec.bonds is not None and len(spec.bonds) > 0:
+            self.set_bonds(spec.bonds, spec.bond_pot, spec.bond_pots, spec.bond_r0)
         if replicas is None:
             self.set_particles(xyz, types)
         else:
@@ -122,6 +124,28 @@ class Simulation:
         if len(xyz) != len(types):
             raise ValueError("xyz and types differ in length")
         self._check(self.lib.rd_set_particles(self.h, len(types), _ptr(xyz), _ptr(types)))
+
+    def set_bonds(self, bonds: np.ndarray, pot_index, pots, r0=None):
+        """Bonded pairs of group potentials (the groups kernel,
+        CudaSimulation.cu:1829-1836): bonds [nb,2] global particle indices,
+        pot_index [nb] (None: all 0) into ``pots`` (OrderTwoPotential), r0 per
+        potential (None or < 0: collision radius sum of the pair's types)."""
+        b = np.ascontiguousarray(bonds, dtype=np.int32).reshape(-1, 2)
+        nb = len(b)
+        pi = (np.zeros(nb, dtype=np.int32) if pot_index is None
+              else np.ascontiguousarray(pot_index, dtype=np.int32).reshape(-1))
+        if len(pi) != nb:
+            raise ValueError("pot_index and bonds differ in length")
+        arr = (_lib.rd_pot2 * max(len(pots), 1))()
+        for i, p in enumerate(pots):
+            arr[i].type, arr[i].subtype, arr[i].force_const = p.kind, p.subtype, p.k
+            arr[i].length, arr[i].depth = p.length, p.depth
+        r = None
+        if r0 is not None:
+            r = np.array([-1.0 if v is None else float(v) for v in r0], dtype=np.float64)
+            if len(r) != len(pots):
+                raise ValueError("r0 needs one value per bond potential")
+        self._check(self.lib.rd_set_bonds(self.h, nb, _ptr(b), _ptr(pi), len(pots), arr, _ptr(r)))
 
     def set_replicas(self, bounds: np.ndarray, counts: np.ndarray, xyz: np.ndarray, types: np.ndarray,
                      streams: Optional[np.ndarray] = None):

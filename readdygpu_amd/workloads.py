@@ -16,8 +16,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from .model import (ATTRACTIVE, CYLINDER, DISK, HARMONIC, PIECEWISE_WELL, REPULSIVE, SPHERE, OrderOnePotential,
-                    OrderTwoPotential, ParticleType, SystemSpec)
+from .model import (ATTRACTIVE, CYLINDER, DISK, HARMONIC, PIECEWISE_WELL, REPULSIVE, SPHERE, SPRING,
+                    OrderOnePotential, OrderTwoPotential, ParticleType, SystemSpec)
 
 
 def uniform_positions(rng: np.random.Generator, n: int, lo, hi) -> np.ndarray:
@@ -180,3 +180,35 @@ def crowded_cytosol(n_replicas: int = 64, seed: int = 5, nmin: int = 20_000, nma
 
 
 CYTOSOL = dict(steps=1000, rdf_range=20.0, rdf_bins=100)
+
+
+def bonded_chains(n_chains: int = 2000, length: int = 8, box: float = 120.0, seed: int = 6, k_bond: float = 50.0):
+    """Linear chains of `length` beads (types alternate 0/1, radii 1.0/1.2 nm,
+    D 1.0/0.8 nm^2/ns) joined by harmonic springs (k_bond, rest length = the
+    pair's collision radius sum), with harmonic repulsion k=10 between all
+    beads, in a periodic cube; dt = 0.01 ns.  Each chain starts as a random
+    walk with steps uniform in [1.8, 2.8] nm (rest length 2.2 nm) from a
+    uniform origin, wrapped into the box, so some bonds cross the periodic
+    boundary.
+    Returns (spec, xyz, types); the bonds are in spec.bonds."""
+    spec = SystemSpec(box_lo=(0.0, 0.0, 0.0), box_hi=(box, box, box), dt=0.01, seed=seed)
+    spec.types = [ParticleType("A", D=1.0, radius=1.0), ParticleType("B", D=0.8, radius=1.2)]
+    spec.pot2 = [OrderTwoPotential(HARMONIC, REPULSIVE, k=10.0)]
+    rng = np.random.default_rng(seed)
+    n = n_chains * length
+    xyz = np.zeros((n, 3))
+    for c in range(n_chains):
+        p = rng.uniform(0.0, box, 3)
+        for b in range(length):
+            if b:
+                v = rng.normal(size=3)
+                p = p + rng.uniform(1.8, 2.8) * v / np.linalg.norm(v)
+            xyz[c * length + b] = p
+    xyz = np.mod(xyz, box).astype(np.float32)
+    xyz = np.minimum(xyz, np.float32(box) - np.float32(1e-3))
+    types = (np.arange(n) % 2).astype(np.int32)
+    first = np.arange(n).reshape(n_chains, length)[:, :-1].reshape(-1)
+    spec.bonds = np.stack([first, first + 1], 1).astype(np.int32)
+    spec.bond_pot = np.zeros(len(first), np.int32)
+    spec.bond_pots = [OrderTwoPotential(HARMONIC, SPRING, k=k_bond)]
+    return spec, xyz, types

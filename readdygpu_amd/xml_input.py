@@ -2,7 +2,9 @@
 (rd_model_read_xml) into a SystemSpec plus particle arrays.
 
 Mirrors Simulation::readXML + readGlobalXML + parse
-(lib/ReaDDyGPU_Cpp/Simulation.cpp:206-338).  Parsing runs on the CPU.
+(lib/ReaDDyGPU_Cpp/Simulation.cpp:206-338), with the bonded pairs of the group
+files (GroupParametersHandler.cpp, GroupReader.cpp:13-40) in
+``spec.bonds``.  Parsing runs on the CPU.
 """
 from __future__ import annotations
 
@@ -75,6 +77,17 @@ def read_xml(paths: dict, max_cutoff: float = 10.0, seed: int = 1) -> XmlInput:
             lib.rd_model_pot2(m, i, C.byref(q), mask.ctypes.data_as(C.c_void_p))
             pairs = [(a, c) for a in range(nt) for c in range(a, nt) if mask[a, c]]
             spec.pot2.append(OrderTwoPotential(q.type, q.subtype, q.force_const, q.length, q.depth, pairs=pairs))
+        if info.n_bonds > 0:
+            nb, npot = info.n_bonds, info.n_bond_pots
+            bonds = np.zeros((nb, 2), np.int32)
+            pot = np.zeros(nb, np.int32)
+            pots = (_lib.rd_pot2 * npot)()
+            r0 = np.zeros(npot)
+            lib.rd_model_bonds(m, bonds.ctypes.data_as(C.c_void_p), pot.ctypes.data_as(C.c_void_p), pots,
+                               r0.ctypes.data_as(C.c_void_p))
+            spec.bonds, spec.bond_pot = bonds, pot
+            spec.bond_pots = [OrderTwoPotential(q.type, q.subtype, q.force_const, q.length, q.depth) for q in pots]
+            spec.bond_r0 = [None if v < 0 else float(v) for v in r0]
         return XmlInput(spec, xyz, types, rid, info.frames, info.steps_per_frame, info.n_groups, info.device)
     finally:
         lib.rd_model_free(m)

@@ -122,6 +122,38 @@ def test_jni_entry_frames_and_reactions(tmp_path, monkeypatch):
     assert np.array_equal(f0[:, :3], xyz)
 
 
+def test_jni_entry_with_bonded_groups(tmp_path, monkeypatch):
+    """The group files of the reference's test_default inputs (a tetramer over
+    particles 0..3, a dimer over 4, 5): the JNI run bonds them exactly as the
+    C ABI run of the parsed model does, and the bonds change the motion."""
+    paths = fixture_dir(tmp_path, frames=1, dt="1e-9")
+    gdir = os.path.join(os.path.dirname(__file__), "golden", "test_default_groups")
+    for f in ("param_groups.xml", "tplgy_groups.xml"):
+        shutil.copy(os.path.join(gdir, f), tmp_path / f)
+    monkeypatch.setenv("RDGPU_SEED", "7")
+    lib = _lib.load()
+    fn = lib.Java_readdy_impl_sim_top_TopGPU_cCreateSimulation
+    fn.restype = None
+    fn.argtypes = [V, V, C.c_uint8] + [V] * 6
+    order = ["param_particles", "tplgy_potentials", "param_groups", "tplgy_coordinates", "tplgy_groups",
+             "param_global"]
+    jvm = MockJVM([paths[k] for k in order], {})
+    fn(V(C.addressof(jvm.env_slot)), V(jvm.OBJ), 0, *[V(100 + i) for i in range(6)])
+    assert not jvm.unexpected and [f for f, _ in jvm.frames] == [0]
+    f0 = jvm.frames[0][1][1:].reshape(-1, 5)
+    x = xml_input.read_xml(paths, seed=7)
+    assert len(x.spec.bonds) == 5
+    with Simulation(x.spec, x.xyz, x.types) as sim:
+        sim.simulate(x.steps_per_frame)
+        xyz, _ = sim.copy_pos_from_device()
+    assert np.array_equal(f0[:, :3], xyz)
+    x.spec.bonds = None
+    with Simulation(x.spec, x.xyz, x.types) as sim:
+        sim.simulate(x.steps_per_frame)
+        free, _ = sim.copy_pos_from_device()
+    assert not np.array_equal(free[:6], xyz[:6])
+
+
 def test_jni_entry_reports_bad_input(tmp_path, capfd):
     paths = fixture_dir(tmp_path, frames=1, dt="1e-9")
     open(paths["tplgy_potentials"], "w").write("<tplgy_potentials><pot type='NOPE'/></tplgy_potentials>")

@@ -6,7 +6,7 @@ import shutil
 import numpy as np
 import pytest
 
-from readdygpu_amd import BOX, HARMONIC, REPULSIVE, ATTRACTIVE
+from readdygpu_amd import BOX, HARMONIC, REPULSIVE, ATTRACTIVE, SPRING
 from readdygpu_amd import xml_input
 
 FIX = os.path.join(os.path.dirname(__file__), "golden", "readdy_gpu_input")
@@ -75,3 +75,34 @@ def test_malformed_and_missing_files_are_errors(lib_built, tmp_path):
     open(tmp_path / "tplgy_coordinates.xml", "a").write('<p id="9" type="7" c="0,0,0"/>\n')
     with pytest.raises(ValueError, match="unknown type"):
         xml_input.read_xml(p)
+
+
+def test_group_files_become_bonds(lib_built, tmp_path):
+    """The reference's test_default group files (test/testInputFiles/
+    test_default): an A_DIMER template (HARMONIC k=1 on internal pair [0,1])
+    and an A_TETRAMER template (HARMONIC k=1 on [0,1];[0,2];[1,3];[2,3]), one
+    instance of each: <g type=1> over particles 0..3, <g type=0> over 4,5.
+    Every affected internal pair of every instance is a bond; no subtype
+    attribute means spring (P2_Harmonic.java:47-49)."""
+    for f in os.listdir(FIX):
+        shutil.copy(os.path.join(FIX, f), tmp_path / f)
+    gdir = os.path.join(os.path.dirname(__file__), "golden", "test_default_groups")
+    for f in ("param_groups.xml", "tplgy_groups.xml"):
+        shutil.copy(os.path.join(gdir, f), tmp_path / f)
+    x = xml_input.read_xml(xml_input.paths_in(str(tmp_path)))
+    s = x.spec
+    assert x.n_groups == 2
+    assert len(s.bond_pots) == 2 and all(p.kind == HARMONIC and p.subtype == SPRING and p.k == 1.0
+                                         for p in s.bond_pots)
+    bonds = sorted((int(a), int(b), int(p)) for (a, b), p in zip(s.bonds, s.bond_pot))
+    assert bonds == [(0, 1, 1), (0, 2, 1), (1, 3, 1), (2, 3, 1), (4, 5, 0)]
+    assert s.bond_r0 == [None, None]
+
+
+def test_group_instance_of_unknown_template_is_an_error(lib_built, tmp_path):
+    for f in os.listdir(FIX):
+        shutil.copy(os.path.join(FIX, f), tmp_path / f)
+    open(tmp_path / "tplgy_groups.xml", "w").write(
+        '<tplgy_groups version="1.1">\n<g id="0" type="0" internalAndParticleId="[[0,0];[1,1]]"/>\n</tplgy_groups>\n')
+    with pytest.raises(ValueError, match="unknown group type"):
+        xml_input.read_xml(xml_input.paths_in(str(tmp_path)))
