@@ -239,6 +239,71 @@ def bench_membrane(steps=200, frames=20):
             "passed": bool(rmse <= 0.01 and dev <= 0.005)}
 
 
+def bench_frames(frames=20, spf=100):
+    """Frame hand-off on the 1M workload: the sequential loop of the
+    reference (simulate, synchronous copy, host packs JPos = [N, (x,y,z,
+    index,id) x N]) against the loop that runs frame f+1 during the hand-off
+    of frame f (pinned asynchronous copy); and the device-side application of
+    a cReactions list (100 removals + 100 creations)."""
+    import numpy as np
+    from readdygpu_amd import Simulation, workloads
+    spec, xyz, types = workloads.plain_1m()
+    n = len(types)
+    ids = np.arange(n, dtype=np.float32)
+
+    def jpos(x):
+        a = np.empty((len(x), 5), np.float32)
+        a[:, :3] = x
+        a[:, 3] = ids[:len(x)]
+        a[:, 4] = ids[:len(x)]
+        return a
+
+    res = {"workload": f"plain_1m, {frames} frames x {spf} steps, JPos packed on the host per frame"}
+    with Simulation(spec, xyz, types) as sim:
+        sim.autotune(trial_steps=2)
+        sim.simulate(spf)
+        t0 = time.perf_counter()
+        sim.simulate(frames * spf)
+        t_gpu = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            sim.simulate(spf)
+            x, _ = sim.copy_pos_from_device()
+            jpos(x)
+        t_seq = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sim.simulate_async(spf)
+        for f in range(frames):
+            sim.frame_copy()
+            if f + 1 < frames:
+                sim.simulate_async(spf)
+            x, _ = sim.frame_wait(copy=False)
+            jpos(x)
+        t_ovl = time.perf_counter() - t0
+        rng = np.random.default_rng(1)
+        ts = []
+        for _ in range(5):
+            m = sim.info().n
+            dele = rng.choice(m, 100, replace=False)
+            e = [200]
+            for i in dele:
+                e += [0, -1, 0, 0, 0, int(i)]
+            for k in range(100):
+                e += [0, k % 4, float(rng.uniform(1, 999)), float(rng.uniform(1, 999)), float(rng.uniform(1, 999)), -1]
+            cr = np.array(e, np.float32)
+            t1 = time.perf_counter()
+            sim.apply_reactions(cr)
+            ts.append(time.perf_counter() - t1)
+    ps = n * spf * frames
+    res.update({"gpu_only_value": ps / t_gpu, "sequential_value": ps / t_seq, "overlapped_value": ps / t_ovl,
+                "unit": "particle-steps/s", "handoff_ms_per_frame_sequential": (t_seq - t_gpu) / frames * 1e3,
+                "handoff_ms_per_frame_overlapped": (t_ovl - t_gpu) / frames * 1e3,
+                "apply_reactions_ms": float(np.median(ts)) * 1e3})
+    log(f"frames: gpu only {res['gpu_only_value']:.3e}, sequential {res['sequential_value']:.3e}, overlapped "
+        f"{res['overlapped_value']:.3e} p-steps/s; reactions {res['apply_reactions_ms']:.2f} ms")
+    return res
+
+
 def bench_cytosol(steps=100, warmup=10, rdf_steps=200, frames=10, plain_steps=20, train_steps=3000):
     """Combined workload: 64 replicas (5.06M particles, phi = 0.3, 8 species,
     two wells, A+B candidates) with a 256-disk membrane acting on the two
@@ -343,7 +408,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
     ap.add_argument("--kernel", type=int, default=-1, help="force kernel: 1 gather")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned workloads")
-    ap.add_argument("--only", choices=["batched_replicas", "learned_slab", "learned_membrane", "crowded_cytosol"],
+    ap.add_argument("--only", choices=["batched_replicas", "learned_slab", "learned_membrane", "crowded_cytosol",
+                                       "frame_handoff"],
                     default=None,
                     help="run only this extra workload and print its JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -357,7 +423,8 @@ def main():
 
     if args.only:
         fn = {"batched_replicas": bench_batched, "learned_slab": bench_learned,
-              "learned_membrane": bench_membrane, "crowded_cytosol": bench_cytosol}[args.only]
+              "learned_membrane": bench_membrane, "crowded_cytosol": bench_cytosol,
+              "frame_handoff": bench_frames}[args.only]
         print(json.dumps(fn()))
         return
     spec, xyz, types = workloads.plain_1m(n=args.n, cell_size=args.cell)
@@ -415,7 +482,8 @@ def main():
     extras = {}
     if not args.no_extras:
         for name, fn in (("batched_replicas", bench_batched), ("learned_slab", bench_learned),
-                         ("learned_membrane", bench_membrane), ("crowded_cytosol", bench_cytosol)):
+                         ("learned_membrane", bench_membrane), ("crowded_cytosol", bench_cytosol),
+                         ("frame_handoff", bench_frames)):
             try:
                 extras[name] = fn()
             except Exception as e:            # reported, not fatal: the headline line still prints

@@ -184,8 +184,27 @@ int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long* count);
 /* ---- reactions: the cReactions protocol of lib/ReaDDyGPU.cpp:303-463:
  *      creactions = [count, (readdyId, newType, x, y, z, index) x count];
  *      index != -1 deletes particle `index`, newType != -1 appends one.
- *      Survivors keep their order; new particles follow. ---- */
+ *      Survivors keep their order; new particles follow.  The compaction runs
+ *      on the device (the reference rebuilds every array on the host and
+ *      uploads it again, CudaSimulation.cu:936-970). ---- */
 int rd_apply_reactions(rd_system* h, const float* creactions);
+
+/* ---- frame hand-off: the frame loop of lib/ReaDDyGPU.cpp:235-301
+ *      (simulate, copyPosFromDevice, frameCallback) without its synchronous
+ *      copies, so the next frame's steps can run during the hand-off ----
+ * rd_simulate_async: enqueue steps and return (errors surface at the next
+ *   rd_frame_wait or rd_simulate).
+ * rd_frame_copy: enqueue a copy of the current positions and types (index
+ *   order) into a device snapshot and into pinned host memory.
+ * rd_frame_wait: wait for that copy; *xyz [3n] and *type [n] stay valid until
+ *   the next rd_frame_copy.
+ * rd_frame_rewind: discard the steps enqueued after the last rd_frame_copy and
+ *   return to its state (positions, types, step counter, so the same noise
+ *   follows); the particle set must be unchanged since the copy. */
+int rd_simulate_async(rd_system* h, int nsteps);
+int rd_frame_copy(rd_system* h);
+int rd_frame_wait(rd_system* h, const float** xyz, const int** type, int* n);
+int rd_frame_rewind(rd_system* h);
 
 int rd_get_info(rd_system* h, rd_info* info);
 

@@ -92,6 +92,23 @@ struct rd_system {
     uint32_t* d_slot = nullptr;       // global index -> slot (null without bonds)
     uint32_t* d_gid = nullptr;        // slot of the next-step array -> global index (null without bonds)
 
+    // device-side reactions: the io arrays hold the new particle set until upload
+    bool io_on_device = false;
+
+    // frame hand-off: the state at the last rd_frame_copy, on the device (for
+    // rd_frame_rewind) and in pinned host memory
+    float* d_snap_xyz = nullptr;
+    int* d_snap_type = nullptr;
+    float* hp_xyz = nullptr;
+    int* hp_type = nullptr;
+    RdStepState* hp_st = nullptr;
+    int snap_cap = 0, snap_n = 0;
+    uint64_t snap_step = 0;
+    bool snap_valid = false;
+    hipEvent_t ev_frame = nullptr;   // copy to the host done (on copy_stream)
+    hipEvent_t ev_snap = nullptr;    // snapshot written (on stream)
+    hipStream_t copy_stream = nullptr;
+
     // RDF
     unsigned long long* d_rdf = nullptr;
     double rdf_rmax = 0.0;

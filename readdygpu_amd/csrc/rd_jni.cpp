@@ -110,23 +110,37 @@ extern "C" void Java_readdy_impl_sim_top_TopGPU_cCreateSimulation(JNIEnv* env, j
     jfieldID fid = fn<GetFieldID_t>(env, RD_JNI_GetFieldID)(env, cls, "cReactions", "[F");
     say("%d frames with each %d steps\n", info.frames, info.steps_per_frame);
 
-    std::vector<float> xyz;
-    std::vector<int> type;
+    // Frames are handed off without stopping the GPU: the positions of frame f
+    // are copied (pinned, asynchronous) while the steps of frame f+1 already
+    // run.  If the Java side then returns reactions for frame f, those steps
+    // are discarded (rd_frame_rewind) and rerun from the changed state, with
+    // the same noise.  Speculation is used while the previous frame had no
+    // reactions, so reactive runs do not pay for discarded frames.
+    const int spf = info.steps_per_frame;
+    bool speculate = true;
+    if (info.frames > 0 && rd_simulate_async(h, spf) != RD_OK) {
+        say("error: %s\n", rd_last_error(h));
+        rd_destroy(h);
+        return;
+    }
     for (int frame = 0; frame < info.frames; ++frame) {
-        if (rd_simulate(h, info.steps_per_frame) != RD_OK) {
+        if (rd_frame_copy(h) != RD_OK) {
+            say("error while copying positions from device: %s\n", rd_last_error(h));
+            break;
+        }
+        const bool ahead = speculate && frame + 1 < info.frames;
+        if (ahead && rd_simulate_async(h, spf) != RD_OK) {
             say("error: %s\n", rd_last_error(h));
             break;
         }
-        say("frame: %d (step %lld)\n", frame, (long long)frame * info.steps_per_frame);
-        rd_info si;
-        rd_get_info(h, &si);
-        const int n = si.n;
-        xyz.resize(3 * (size_t)n);
-        type.resize(n);
-        if (rd_copy_pos_from_device(h, xyz.data(), type.data()) != RD_OK) {
-            say("error while copying positions from device\n");
+        const float* xyz = nullptr;
+        const int* type = nullptr;
+        int n = 0;
+        if (rd_frame_wait(h, &xyz, &type, &n) != RD_OK) {
+            say("error: %s\n", rd_last_error(h));
             break;
         }
+        say("frame: %d (step %lld)\n", frame, (long long)frame * spf);
         // JPos = [N, (x, y, z, index, readdyId) x N]
         jfloatArray jpos = fn<NewFloatArray_t>(env, RD_JNI_NewFloatArray)(env, (jsize)(5 * n + 1));
         float* pos = jpos ? (float*)fn<GetPrimitiveArrayCritical_t>(env, RD_JNI_GetPrimitiveArrayCritical)(env, jpos, nullptr)
@@ -151,20 +165,34 @@ extern "C" void Java_readdy_impl_sim_top_TopGPU_cCreateSimulation(JNIEnv* env, j
             break;
         }
         // reactions from the Java side
-        if (!fid) continue;
-        jobject arr = fn<GetObjectField_t>(env, RD_JNI_GetObjectField)(env, obj, fid);
-        if (!arr) continue;
-        jsize len = fn<GetArrayLength_t>(env, RD_JNI_GetArrayLength)(env, arr);
         std::vector<float> cr;
-        float* r = (float*)fn<GetPrimitiveArrayCritical_t>(env, RD_JNI_GetPrimitiveArrayCritical)(env, arr, nullptr);
-        if (r && len > 0) {
-            int cnt = (int)r[0];
-            if (cnt > 0 && 1 + 6 * (long)cnt <= len) cr.assign(r, r + 1 + 6 * (size_t)cnt);
-            else if (cnt > 0) say("error: cReactions shorter than its count\n");
+        if (fid) {
+            jobject arr = fn<GetObjectField_t>(env, RD_JNI_GetObjectField)(env, obj, fid);
+            if (arr) {
+                jsize len = fn<GetArrayLength_t>(env, RD_JNI_GetArrayLength)(env, arr);
+                float* r = (float*)fn<GetPrimitiveArrayCritical_t>(env, RD_JNI_GetPrimitiveArrayCritical)(env, arr, nullptr);
+                if (r && len > 0) {
+                    int cnt = (int)r[0];
+                    if (cnt > 0 && 1 + 6 * (long)cnt <= len) cr.assign(r, r + 1 + 6 * (size_t)cnt);
+                    else if (cnt > 0) say("error: cReactions shorter than its count\n");
+                }
+                if (r) fn<ReleasePrimitiveArrayCritical_t>(env, RD_JNI_ReleasePrimitiveArrayCritical)(env, arr, r, 0);
+                fn<DeleteLocalRef_t>(env, RD_JNI_DeleteLocalRef)(env, arr);
+            }
         }
-        if (r) fn<ReleasePrimitiveArrayCritical_t>(env, RD_JNI_ReleasePrimitiveArrayCritical)(env, arr, r, 0);
-        fn<DeleteLocalRef_t>(env, RD_JNI_DeleteLocalRef)(env, arr);
-        if (cr.empty()) continue;
+        if (cr.empty()) {
+            speculate = true;
+            if (!ahead && frame + 1 < info.frames && rd_simulate_async(h, spf) != RD_OK) {
+                say("error: %s\n", rd_last_error(h));
+                break;
+            }
+            continue;
+        }
+        speculate = false;
+        if (ahead && rd_frame_rewind(h) != RD_OK) {
+            say("error: %s\n", rd_last_error(h));
+            break;
+        }
         // same compaction as rd_apply_reactions, applied to the ReaDDy ids
         std::vector<int> keep(rid.begin(), rid.end());
         std::vector<char> dead(n, 0);
@@ -184,6 +212,10 @@ extern "C" void Java_readdy_impl_sim_top_TopGPU_cCreateSimulation(JNIEnv* env, j
         for (int j = 0; j < n; ++j)
             if (!dead[j]) rid.push_back(keep[j]);
         rid.insert(rid.end(), born.begin(), born.end());
+        if (frame + 1 < info.frames && rd_simulate_async(h, spf) != RD_OK) {
+            say("error: %s\n", rd_last_error(h));
+            break;
+        }
     }
     rd_destroy(h);
 }

@@ -246,6 +246,33 @@ __global__ __launch_bounds__(RD_BLOCK) void k_unsort(const float4* __restrict__ 
     type[i] = w_type(p.w);
 }
 
+// Reaction compaction (the cReactions protocol, lib/ReaDDyGPU.cpp:303-463,
+// on the device): survivors keep their order, so the new index of particle i
+// is i minus the number of removed indices below it (binary search in the
+// sorted removal list); removed particles are dropped.
+__global__ __launch_bounds__(RD_BLOCK) void k_react_compact(const float4* __restrict__ ps,
+                                                            const RdRep* __restrict__ reps,
+                                                            const RdWave* __restrict__ waves, int nwaves,
+                                                            const int* __restrict__ del, int ndel,
+                                                            float* __restrict__ xyz, int* __restrict__ type)
+{
+    const RdWave it = wave_item(waves, nwaves, wave_id());
+    const int s = it.s0 + (threadIdx.x & 63);
+    if (s >= it.s1) return;
+    const float4 p = ps[s];
+    const int gi = reps[it.rep].off + (int)w_index(p.w);
+    int lo = 0, hi = ndel;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (del[mid] < gi) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < ndel && del[lo] == gi) return;
+    const size_t ni = (size_t)(gi - lo);
+    xyz[3 * ni] = p.x; xyz[3 * ni + 1] = p.y; xyz[3 * ni + 2] = p.z;
+    type[ni] = w_type(p.w);
+}
+
 // reference cell index (replica-local) per particle
 __global__ __launch_bounds__(RD_BLOCK) void k_cell_ids(const float4* __restrict__ ps, const RdRep* __restrict__ reps,
                                                        const RdWave* __restrict__ waves, int nwaves,
@@ -408,9 +435,26 @@ static void free_bonds(rd_system* h)
     h->d_bstart = nullptr; h->d_bonds = nullptr; h->d_slot = nullptr; h->d_gid = nullptr;
 }
 
+static void free_frames(rd_system* h)
+{
+    if (h->d_snap_xyz) (void)hipFree(h->d_snap_xyz);
+    if (h->d_snap_type) (void)hipFree(h->d_snap_type);
+    if (h->hp_xyz) (void)hipHostFree(h->hp_xyz);
+    if (h->hp_type) (void)hipHostFree(h->hp_type);
+    if (h->hp_st) (void)hipHostFree(h->hp_st);
+    if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
+    if (h->ev_frame) (void)hipEventDestroy(h->ev_frame);
+    if (h->ev_snap) (void)hipEventDestroy(h->ev_snap);
+    if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
+    h->d_snap_xyz = nullptr; h->d_snap_type = nullptr; h->hp_xyz = nullptr; h->hp_type = nullptr;
+    h->hp_st = nullptr; h->ev_frame = nullptr; h->ev_snap = nullptr; h->copy_stream = nullptr;
+    h->snap_cap = 0; h->snap_valid = false;
+}
+
 static void free_dev(rd_system* h)
 {
     free_bonds(h);
+    free_frames(h);
     void* ptrs[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_bsum,
                      h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_pairs, h->d_pcount,
                      h->d_xyz_io, h->d_type_io, h->d_etot, h->d_reps, h->d_waves, h->d_rep_of };
@@ -749,16 +793,25 @@ extern "C" int rd_set_replicas(rd_system* h, int nrep, const double* bounds, con
     return RD_OK;
 }
 
-static int ensure_capacity(rd_system* h)
+// per-particle arrays for at least `need` particles; with keep_ps the sorted
+// state (h->n slots) survives the move
+static int grow_particles(rd_system* h, int need, bool keep_ps)
 {
-    int n = std::max(h->n, 1);
-    if (n > h->cap) {
-        drop_graph(h);
-        int cap = std::max(n, h->cap + h->cap / 2);
+    drop_graph(h);
+    int cap = std::max(need, h->cap + h->cap / 2);
+    float4* nps = nullptr;
+    HIPCK(h, hipMalloc(&nps, sizeof(float4) * cap));
+    if (keep_ps && h->d_ps && h->n > 0) {
+        HIPCK(h, hipMemcpyAsync(nps, h->d_ps, sizeof(float4) * (size_t)h->n, hipMemcpyDeviceToDevice, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+    }
+    {
         void* ps[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_xyz_io, h->d_type_io, h->d_rep_of };
         for (void* p : ps)
             if (p) (void)hipFree(p);
-        HIPCK(h, hipMalloc(&h->d_ps, sizeof(float4) * cap));
+        h->d_pn = nullptr; h->d_key = h->d_rank = nullptr; h->d_xyz_io = nullptr; h->d_type_io = nullptr;
+        h->d_rep_of = nullptr;
+        h->d_ps = nps;
         HIPCK(h, hipMalloc(&h->d_pn, sizeof(float4) * cap));
         HIPCK(h, hipMalloc(&h->d_key, sizeof(uint32_t) * cap));
         HIPCK(h, hipMalloc(&h->d_rank, sizeof(uint32_t) * cap));
@@ -766,6 +819,16 @@ static int ensure_capacity(rd_system* h)
         HIPCK(h, hipMalloc(&h->d_type_io, sizeof(int) * cap));
         HIPCK(h, hipMalloc(&h->d_rep_of, sizeof(int) * cap));
         h->cap = cap;
+    }
+    return RD_OK;
+}
+
+static int ensure_capacity(rd_system* h)
+{
+    int n = std::max(h->n, 1);
+    if (n > h->cap) {
+        int rc = grow_particles(h, n, false);
+        if (rc) return rc;
     }
     if (h->ncell != h->ncell_alloc) {
         drop_graph(h);
@@ -923,8 +986,12 @@ static int upload(rd_system* h)
         h->hrep_of.resize(h->n);
         for (size_t r = 0; r < h->rep_dev.size(); ++r)
             for (int i = 0; i < h->rep_dev[r].n; ++i) h->hrep_of[h->rep_dev[r].off + i] = (int)r;
-        HIPCK(h, hipMemcpyAsync(h->d_xyz_io, h->hxyz.data(), sizeof(float) * 3 * h->n, hipMemcpyHostToDevice, h->stream));
-        HIPCK(h, hipMemcpyAsync(h->d_type_io, h->htype.data(), sizeof(int) * h->n, hipMemcpyHostToDevice, h->stream));
+        if (!h->io_on_device) {
+            HIPCK(h, hipMemcpyAsync(h->d_xyz_io, h->hxyz.data(), sizeof(float) * 3 * h->n, hipMemcpyHostToDevice,
+                                    h->stream));
+            HIPCK(h, hipMemcpyAsync(h->d_type_io, h->htype.data(), sizeof(int) * h->n, hipMemcpyHostToDevice,
+                                    h->stream));
+        }
         HIPCK(h, hipMemcpyAsync(h->d_rep_of, h->hrep_of.data(), sizeof(int) * h->n, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io, h->d_type_io,
                            h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_gid);
@@ -1325,6 +1392,13 @@ extern "C" int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long
 }
 
 // ---- reactions: lib/ReaDDyGPU.cpp:303-463 (single system) ----
+// cReactions = [count, (readdyId, newType, x, y, z, index) x count]
+// (lib/ReaDDyGPU.cpp:303-365): index != -1 removes that particle, newType != -1
+// appends a particle of that type at (x, y, z); both is a type change (the
+// particle is removed and its successor appended).  Survivors keep their
+// order, new particles follow in entry order.  Applied on the device: the
+// sorted state is compacted into index order, new particles are copied in,
+// and the cell list is rebuilt; no particle data crosses PCIe.
 extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
 {
     if (!h || !cr) return RD_EINVAL;
@@ -1332,49 +1406,186 @@ extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
     if (h->reps.size() != 1) return set_err(h, RD_ESTATE, "reactions need a single-system handle");
     int nr = (int)cr[0];
     if (nr <= 0) return RD_OK;
-    int rc = rd_copy_pos_from_device(h, nullptr, nullptr);
+    int rc = refresh(h);
     if (rc) return rc;
-    std::vector<float> nx(h->hxyz);
-    std::vector<int> ntp(h->htype);
+    const int n = h->n;
+    std::vector<int> del;
+    std::vector<float> cx;
+    std::vector<int> ct;
     for (int r = 0; r < nr; ++r) {
         const float* e = &cr[1 + 6 * r];
         int newType = (int)e[1];
         int index = (int)e[5];
         if (index != -1) {
-            if (index < 0 || index >= h->n || ntp[index] == -1)
-                return set_err(h, RD_EINVAL, "reaction deletes a nonexistent particle");
-            ntp[index] = -1;
+            if (index < 0 || index >= n) return set_err(h, RD_EINVAL, "reaction deletes a nonexistent particle");
+            del.push_back(index);
         }
         if (newType != -1) {
             if (newType < 0 || newType >= h->nt) return set_err(h, RD_EINVAL, "reaction creates an unknown type");
-            nx.push_back(e[2]); nx.push_back(e[3]); nx.push_back(e[4]);
-            ntp.push_back(newType);
+            cx.push_back(e[2]); cx.push_back(e[3]); cx.push_back(e[4]);
+            ct.push_back(newType);
         }
     }
-    std::vector<float> cx;
-    std::vector<int> ct;
-    std::vector<int> remap(ntp.size(), -1);
-    for (size_t i = 0; i < ntp.size(); ++i)
-        if (ntp[i] != -1) {
-            remap[i] = (int)ct.size();
-            cx.push_back(nx[3 * i]); cx.push_back(nx[3 * i + 1]); cx.push_back(nx[3 * i + 2]);
-            ct.push_back(ntp[i]);
+    std::sort(del.begin(), del.end());
+    if (std::adjacent_find(del.begin(), del.end()) != del.end())
+        return set_err(h, RD_EINVAL, "reaction deletes a nonexistent particle");
+    const rd_system::Rep& A = h->reps[0];
+    rc = validate_particles(h, (int)ct.size(), cx.data(), ct.data(), A.lo, A.hi, n - (int)del.size());
+    if (rc) return rc;
+    const int nkeep = n - (int)del.size();
+    const int n_new = nkeep + (int)ct.size();
+    if (n_new > RD_MAX_PARTICLES) return set_err(h, RD_EINVAL, "too many particles");
+    if (n_new > h->cap) {
+        rc = grow_particles(h, n_new, true);
+        if (rc) return rc;
+    }
+    int* d_del = nullptr;
+    if (!del.empty()) {
+        HIPCK(h, hipMalloc(&d_del, sizeof(int) * del.size()));
+        HIPCK(h, hipMemcpyAsync(d_del, del.data(), sizeof(int) * del.size(), hipMemcpyHostToDevice, h->stream));
+    }
+    if (n > 0)
+        hipLaunchKernelGGL(k_react_compact, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_reps,
+                           h->d_waves, h->nwaves, d_del, (int)del.size(), h->d_xyz_io, h->d_type_io);
+    if (!ct.empty()) {
+        HIPCK(h, hipMemcpyAsync(h->d_xyz_io + 3 * (size_t)nkeep, cx.data(), sizeof(float) * cx.size(),
+                                hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->d_type_io + nkeep, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice,
+                                h->stream));
+    }
+    hipError_t ke = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(h->stream);
+    if (d_del) (void)hipFree(d_del);
+    if (ke != hipSuccess || se != hipSuccess)
+        return set_err(h, RD_EHIP, std::string("reaction compaction: ") + hipGetErrorString(ke != hipSuccess ? ke : se));
+    // host mirrors: types follow the compaction; positions live on the device
+    auto new_index = [&](int i) {
+        auto it = std::lower_bound(del.begin(), del.end(), i);
+        return (it != del.end() && *it == i) ? -1 : i - (int)(it - del.begin());
+    };
+    {
+        std::vector<int> nt;
+        nt.reserve(n_new);
+        size_t k = 0;
+        for (int i = 0; i < n; ++i) {
+            if (k < del.size() && del[k] == i) { ++k; continue; }
+            nt.push_back(h->htype[i]);
         }
+        nt.insert(nt.end(), ct.begin(), ct.end());
+        h->htype.swap(nt);
+        h->hxyz.resize(3 * (size_t)n_new);
+    }
     // bonds follow their particles; a bond of a removed particle is dropped
     if (!h->bond_pot.empty()) {
         std::vector<int> bp, bo;
-        auto rm = [&](int i) { return i < (int)remap.size() ? remap[i] : -1; };
         for (size_t b = 0; b < h->bond_pot.size(); ++b) {
-            const int a = rm(h->bond_pairs[2 * b]), c = rm(h->bond_pairs[2 * b + 1]);
+            const int a0 = h->bond_pairs[2 * b], c0 = h->bond_pairs[2 * b + 1];
+            const int a = a0 < n ? new_index(a0) : -1, c = c0 < n ? new_index(c0) : -1;
             if (a < 0 || c < 0) continue;
             bp.push_back(a); bp.push_back(c); bo.push_back(h->bond_pot[b]);
         }
         h->bond_pairs.swap(bp);
         h->bond_pot.swap(bo);
     }
-    rc = rd_set_particles(h, (int)ct.size(), cx.data(), ct.data());
+    h->n = n_new;
+    h->reps[0].n = n_new;
+    h->snap_valid = false;
+    h->io_on_device = true;
+    rc = upload(h);
+    h->io_on_device = false;
+    return rc;
+}
+
+// ---- frame hand-off (the frame loop of lib/ReaDDyGPU.cpp:235-301) ----
+extern "C" int rd_simulate_async(rd_system* h, int nsteps)
+{
+    if (!h || nsteps < 0) return RD_EINVAL;
+    int rc = refresh(h);
     if (rc) return rc;
-    return refresh(h);
+    return nsteps > 0 ? launch_steps(h, nsteps) : RD_OK;
+}
+
+extern "C" int rd_frame_copy(rd_system* h)
+{
+    if (!h) return RD_EINVAL;
+    int rc = refresh(h);
+    if (rc) return rc;
+    const int n = std::max(h->n, 1);
+    if (n > h->snap_cap) {
+        const int cap = std::max(n, h->snap_cap + h->snap_cap / 2);
+        free_frames(h);
+        HIPCK(h, hipMalloc(&h->d_snap_xyz, sizeof(float) * 3 * (size_t)cap));
+        HIPCK(h, hipMalloc(&h->d_snap_type, sizeof(int) * (size_t)cap));
+        HIPCK(h, hipHostMalloc((void**)&h->hp_xyz, sizeof(float) * 3 * (size_t)cap, hipHostMallocDefault));
+        HIPCK(h, hipHostMalloc((void**)&h->hp_type, sizeof(int) * (size_t)cap, hipHostMallocDefault));
+        HIPCK(h, hipHostMalloc((void**)&h->hp_st, sizeof(RdStepState), hipHostMallocDefault));
+        HIPCK(h, hipEventCreateWithFlags(&h->ev_frame, hipEventDisableTiming));
+        HIPCK(h, hipEventCreateWithFlags(&h->ev_snap, hipEventDisableTiming));
+        HIPCK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+        h->snap_cap = cap;
+    } else if (h->snap_valid) {
+        // the previous copy to the host must be done before the snapshot is overwritten
+        HIPCK(h, hipStreamWaitEvent(h->stream, h->ev_frame, 0));
+    }
+    // snapshot on the step stream; the copy to the host runs on its own stream
+    // so the steps enqueued next do not wait for it
+    if (h->n > 0) {
+        hipLaunchKernelGGL(k_unsort, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_reps,
+                           h->d_waves, h->nwaves, h->d_snap_xyz, h->d_snap_type);
+        HIPCK(h, hipGetLastError());
+    }
+    HIPCK(h, hipMemcpyAsync(h->hp_st, h->d_st, sizeof(RdStepState), hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipEventRecord(h->ev_snap, h->stream));
+    HIPCK(h, hipStreamWaitEvent(h->copy_stream, h->ev_snap, 0));
+    if (h->n > 0) {
+        HIPCK(h, hipMemcpyAsync(h->hp_xyz, h->d_snap_xyz, sizeof(float) * 3 * (size_t)h->n, hipMemcpyDeviceToHost,
+                                h->copy_stream));
+        HIPCK(h, hipMemcpyAsync(h->hp_type, h->d_snap_type, sizeof(int) * (size_t)h->n, hipMemcpyDeviceToHost,
+                                h->copy_stream));
+    }
+    HIPCK(h, hipEventRecord(h->ev_frame, h->copy_stream));
+    h->snap_n = h->n;
+    h->snap_step = h->step;
+    h->snap_valid = true;
+    return RD_OK;
+}
+
+extern "C" int rd_frame_wait(rd_system* h, const float** xyz, const int** type, int* n)
+{
+    if (!h) return RD_EINVAL;
+    if (!h->snap_valid) return set_err(h, RD_ESTATE, "no frame copy in flight");
+    HIPCK(h, hipEventSynchronize(h->ev_frame));
+    if (h->hp_st->err & 1u) return set_err(h, RD_ERANGE, "non-finite particle position");
+    if (h->hp_st->err & 2u) return set_err(h, RD_ERANGE, "noise array exhausted");
+    if (xyz) *xyz = h->hp_xyz;
+    if (type) *type = h->hp_type;
+    if (n) *n = h->snap_n;
+    return RD_OK;
+}
+
+extern "C" int rd_frame_rewind(rd_system* h)
+{
+    if (!h) return RD_EINVAL;
+    if (!h->snap_valid || h->snap_n != h->n) return set_err(h, RD_ESTATE, "no frame copy of the current particle set");
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    if (h->n > 0) {
+        HIPCK(h, hipMemcpyAsync(h->d_xyz_io, h->d_snap_xyz, sizeof(float) * 3 * (size_t)h->n,
+                                hipMemcpyDeviceToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->d_type_io, h->d_snap_type, sizeof(int) * (size_t)h->n, hipMemcpyDeviceToDevice,
+                                h->stream));
+    }
+    int rc = rd_host_set_step(h, h->snap_step);
+    if (rc) return rc;
+    // same particle set and tables: pack the snapshot and sort it again (the
+    // captured step graphs stay valid)
+    if (h->n > 0)
+        hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io,
+                           h->d_type_io, h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count,
+                           h->d_gid);
+    launch_sort(h, 0);
+    HIPCK(h, hipGetLastError());
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    return RD_OK;
 }
 
 extern "C" int rd_get_info(rd_system* h, rd_info* info)

@@ -246,6 +246,33 @@ class Simulation:
     def reaction_candidates(self) -> np.ndarray:
         return self._pairs(self.lib.rd_reaction_candidates)
 
+    def simulate_async(self, nsteps: int):
+        """Enqueue steps without waiting (errors surface at frame_wait)."""
+        self._check(self.lib.rd_simulate_async(self.h, int(nsteps)))
+
+    def frame_copy(self):
+        """Start copying the current positions/types (index order) to the host."""
+        self._check(self.lib.rd_frame_copy(self.h))
+
+    def frame_wait(self, copy: bool = True):
+        """Positions [N,3] and types [N] of the last frame_copy; copy=False
+        returns views of the pinned host buffers, valid until the next
+        frame_copy."""
+        px = C.POINTER(C.c_float)()
+        pt = C.POINTER(C.c_int)()
+        n = C.c_int()
+        self._check(self.lib.rd_frame_wait(self.h, C.byref(px), C.byref(pt), C.byref(n)))
+        k = n.value
+        if k == 0:
+            return np.zeros((0, 3), np.float32), np.zeros(0, np.int32)
+        xyz = np.ctypeslib.as_array(px, shape=(3 * k,)).reshape(k, 3)
+        types = np.ctypeslib.as_array(pt, shape=(k,))
+        return (xyz.copy(), types.copy()) if copy else (xyz, types)
+
+    def frame_rewind(self):
+        """Back to the state of the last frame_copy (same noise follows)."""
+        self._check(self.lib.rd_frame_rewind(self.h))
+
     def apply_reactions(self, creactions: np.ndarray):
         """cReactions protocol (lib/ReaDDyGPU.cpp:303-463)."""
         cr = np.ascontiguousarray(creactions, dtype=np.float32)

@@ -114,12 +114,17 @@ def test_jni_entry_frames_and_reactions(tmp_path, monkeypatch):
     ids = f2[1:].reshape(-1, 5)[:, 4].astype(int)
     assert 0 not in ids and ids[-1] == 5000 and list(ids[:-1]) == list(range(1, 800))
 
-    # frame 0 equals 10 steps of the same model through the C ABI (same seed)
+    # every frame equals the sequential C-ABI loop (simulate, copy, react) with
+    # the same seed: the JNI loop runs frame f+1 during the hand-off of frame f
+    # and rewinds it when reactions arrive
     x = xml_input.read_xml(paths, seed=5)
     with Simulation(x.spec, x.xyz, x.types) as sim:
-        sim.simulate(x.steps_per_frame)
-        xyz, _ = sim.copy_pos_from_device()
-    assert np.array_equal(f0[:, :3], xyz)
+        for f in range(3):
+            sim.simulate(x.steps_per_frame)
+            xyz, _ = sim.copy_pos_from_device()
+            assert np.array_equal(jvm.frames[f][1][1:].reshape(-1, 5)[:, :3], xyz), f"frame {f}"
+            if f == 1:
+                sim.apply_reactions(np.array([1, 5000, 1, 1.0, 2.0, 3.0, 0], np.float32))
 
 
 def test_jni_entry_with_bonded_groups(tmp_path, monkeypatch):
