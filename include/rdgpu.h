@@ -189,6 +189,22 @@ int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long* count);
  *      uploads it again, CudaSimulation.cu:936-970). ---- */
 int rd_apply_reactions(rd_system* h, const float* creactions);
 
+/* ---- output files of the reference's stand-alone driver
+ *      (outputHandler.cpp:17-148, opened in lib/ReaDDyGPU_Cpp/ReaDDyGPU.cpp:68-85):
+ *      the same text, printf format for printf format; the XML trajectory is
+ *      closed with "</traj>" (the reference writes "<" TAB "raj>").  No GPU
+ *      needed. ---- */
+enum { RD_TRAJ_PDB = 1, RD_TRAJ_XML = 2 };
+typedef struct rd_traj rd_traj;
+int rd_traj_open(const char* path, int format, rd_traj** out);
+/* one MODEL / tplgy_coords block: particle a is written as id a+1 */
+int rd_traj_write_frame(rd_traj* t, int frame, int n, const float* xyz, const int* type);
+int rd_traj_close(rd_traj* t);
+/* RDF table: header "d" + ",\tRDF(a,b)" per type pair, then per bin k the
+ * distance k*range/nbins ("%8.5f") and g[(i*nt+j)*nbins+k] (", %8.3f") */
+int rd_rdf_write(const char* path, int n_types, const char* const* type_names, int nbins, double range,
+                 const double* g);
+
 /* ---- frame hand-off: the frame loop of lib/ReaDDyGPU.cpp:235-301
  *      (simulate, copyPosFromDevice, frameCallback) without its synchronous
  *      copies, so the next frame's steps can run during the hand-off ----

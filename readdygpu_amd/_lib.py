@@ -117,6 +117,10 @@ _SIGS = {
     "rd_add_order_two_potential": (C.c_int, [P, C.POINTER(rd_pot2), P]),
     "rd_set_reaction_radii": (C.c_int, [P, P]),
     "rd_set_bonds": (C.c_int, [P, C.c_int, P, P, C.c_int, C.POINTER(rd_pot2), P]),
+    "rd_traj_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(P)]),
+    "rd_traj_write_frame": (C.c_int, [P, C.c_int, C.c_int, P, P]),
+    "rd_traj_close": (C.c_int, [P]),
+    "rd_rdf_write": (C.c_int, [C.c_char_p, C.c_int, P, C.c_int, C.c_double, P]),
     "rd_simulate_async": (C.c_int, [P, C.c_int]),
     "rd_frame_copy": (C.c_int, [P]),
     "rd_frame_wait": (C.c_int, [P, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]),

@@ -116,6 +116,22 @@ extern "C" void Java_readdy_impl_sim_top_TopGPU_cCreateSimulation(JNIEnv* env, j
     // are discarded (rd_frame_rewind) and rerun from the changed state, with
     // the same noise.  Speculation is used while the previous frame had no
     // reactions, so reactive runs do not pay for discarded frames.
+    // Trajectory files: the reference's JNI entry has them switched off
+    // (output = false, lib/ReaDDyGPU.cpp:123) and its stand-alone driver writes
+    // ReaDDy_output/pdb.pdb and out_traj.xml per frame; RDGPU_OUTPUT=<dir>
+    // turns them on here, written while the next frame runs.
+    rd_traj* pdb = nullptr;
+    rd_traj* xmlt = nullptr;
+    if (const char* od = getenv("RDGPU_OUTPUT")) {
+        std::string d(od);
+        if (rd_traj_open((d + "/pdb.pdb").c_str(), RD_TRAJ_PDB, &pdb) != RD_OK ||
+            rd_traj_open((d + "/out_traj.xml").c_str(), RD_TRAJ_XML, &xmlt) != RD_OK) {
+            say("error while opening trajectory output files\n");
+            if (pdb) rd_traj_close(pdb);
+            rd_destroy(h);
+            return;
+        }
+    }
     const int spf = info.steps_per_frame;
     bool speculate = true;
     if (info.frames > 0 && rd_simulate_async(h, spf) != RD_OK) {
@@ -141,6 +157,11 @@ extern "C" void Java_readdy_impl_sim_top_TopGPU_cCreateSimulation(JNIEnv* env, j
             break;
         }
         say("frame: %d (step %lld)\n", frame, (long long)frame * spf);
+        if (pdb && (rd_traj_write_frame(pdb, frame, n, xyz, type) != RD_OK ||
+                    rd_traj_write_frame(xmlt, frame, n, xyz, type) != RD_OK)) {
+            say(" error while writing trajectory output\n");
+            break;
+        }
         // JPos = [N, (x, y, z, index, readdyId) x N]
         jfloatArray jpos = fn<NewFloatArray_t>(env, RD_JNI_NewFloatArray)(env, (jsize)(5 * n + 1));
         float* pos = jpos ? (float*)fn<GetPrimitiveArrayCritical_t>(env, RD_JNI_GetPrimitiveArrayCritical)(env, jpos, nullptr)
@@ -217,5 +238,7 @@ extern "C" void Java_readdy_impl_sim_top_TopGPU_cCreateSimulation(JNIEnv* env, j
             break;
         }
     }
+    if (pdb) rd_traj_close(pdb);
+    if (xmlt) rd_traj_close(xmlt);
     rd_destroy(h);
 }

@@ -8,7 +8,7 @@ import shutil
 import numpy as np
 import pytest
 
-from readdygpu_amd import Simulation, _lib, xml_input
+from readdygpu_amd import Simulation, _lib, output, xml_input
 
 pytestmark = pytest.mark.gpu
 FIX = os.path.join(os.path.dirname(__file__), "golden", "readdy_gpu_input")
@@ -136,6 +136,9 @@ def test_jni_entry_with_bonded_groups(tmp_path, monkeypatch):
     for f in ("param_groups.xml", "tplgy_groups.xml"):
         shutil.copy(os.path.join(gdir, f), tmp_path / f)
     monkeypatch.setenv("RDGPU_SEED", "7")
+    outdir = tmp_path / "out"
+    outdir.mkdir()
+    monkeypatch.setenv("RDGPU_OUTPUT", str(outdir))
     lib = _lib.load()
     fn = lib.Java_readdy_impl_sim_top_TopGPU_cCreateSimulation
     fn.restype = None
@@ -152,6 +155,12 @@ def test_jni_entry_with_bonded_groups(tmp_path, monkeypatch):
         sim.simulate(x.steps_per_frame)
         xyz, _ = sim.copy_pos_from_device()
     assert np.array_equal(f0[:, :3], xyz)
+    # trajectory files of the frame, as the stand-alone driver writes them
+    for name, fmt in (("pdb.pdb", output.PDB), ("out_traj.xml", output.XML)):
+        ref = str(tmp_path / ("ref_" + name))
+        with output.TrajectoryWriter(ref, fmt) as w:
+            w.write_frame(0, xyz, x.types)
+        assert open(outdir / name).read() == open(ref).read()
     x.spec.bonds = None
     with Simulation(x.spec, x.xyz, x.types) as sim:
         sim.simulate(x.steps_per_frame)
