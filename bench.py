@@ -304,6 +304,79 @@ def bench_frames(frames=20, spf=100):
     return res
 
 
+def cytosol_reactions(spec, boxes, counts, xyz, types, pieces, mtypes, w, norm, frames=10, spf=100, p=0.2,
+                      c_type=2):
+    """A + B -> C at 3 nm in the batched cytosol with the surrogate membrane:
+    every spf steps the candidate pairs (species 0 and 1 within 3 nm) react
+    with probability p; a particle reacts at most once per frame (an accepted
+    pair reacts when its random priority is the lowest at both particles); A
+    is removed and B becomes C (species c_type) at the pair's midpoint, in its
+    replica."""
+    import numpy as np
+    from readdygpu_amd import Simulation
+    rng = np.random.default_rng(55)
+    cnt = np.array(counts, dtype=np.int64)
+    sim = Simulation(spec, xyz, types, replicas=(boxes, counts))
+    sim.set_composite(pieces, types=mtypes)
+    sim.set_surrogate_weights(w)
+    sim.set_surrogate_norm(*norm)
+    sim.set_external_mode(True)
+    sim.simulate(10)
+    nreact, t_cand, t_dec, t_apply = 0, 0.0, 0.0, 0.0
+    lo_all, hi_all = boxes[:, 0::2], boxes[:, 1::2]
+    t0 = time.perf_counter()
+    for f in range(frames):
+        sim.simulate(spf)
+        t1 = time.perf_counter()
+        c = sim.reaction_candidates()
+        sim.frame_copy()
+        x, t = sim.frame_wait(copy=False)
+        t2 = time.perf_counter()
+        acc = c[rng.random(len(c)) < p]
+        pri = rng.random(len(acc))
+        best = np.full(len(t), np.inf)
+        np.minimum.at(best, acc[:, 0], pri)
+        np.minimum.at(best, acc[:, 1], pri)
+        sel = acc[(best[acc[:, 0]] == pri) & (best[acc[:, 1]] == pri)]
+        a = np.where(t[sel[:, 0]] == 0, sel[:, 0], sel[:, 1])
+        b = np.where(t[sel[:, 0]] == 0, sel[:, 1], sel[:, 0])
+        offs = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+        r = np.searchsorted(offs, a, side="right") - 1
+        lo, hi = lo_all[r], hi_all[r]
+        L = hi - lo
+        d = x[b].astype(np.float64) - x[a]
+        d -= L * np.round(d / L)
+        mid = np.minimum(np.mod(x[a] + 0.5 * d - lo, L) + lo, hi - 1e-3)
+        k = len(sel)
+        ent = np.zeros((k, 2, 6), np.float32)
+        ent[:, 0, 1] = -1
+        ent[:, 0, 5] = a
+        ent[:, 1, 1] = c_type
+        ent[:, 1, 2:5] = mid
+        ent[:, 1, 5] = b
+        np.subtract.at(cnt, r, 1)
+        t3 = time.perf_counter()
+        if k:
+            sim.apply_reactions(np.concatenate([[2 * k], ent.reshape(-1)]).astype(np.float32),
+                                creation_replica=np.repeat(r, 2).astype(np.int32))
+        t4 = time.perf_counter()
+        nreact += k
+        t_cand += t2 - t1
+        t_dec += t3 - t2
+        t_apply += t4 - t3
+    wall = time.perf_counter() - t0
+    info = sim.info()
+    sim.close()
+    ps = float(np.sum(counts)) * frames * spf
+    out = {"frames": frames, "steps_per_frame": spf, "reactions": nreact, "n_after": info.n,
+           "value_incl_reactions": ps / wall, "candidates_and_copy_ms_per_frame": t_cand / frames * 1e3,
+           "host_decision_ms_per_frame": t_dec / frames * 1e3, "apply_ms_per_frame": t_apply / frames * 1e3}
+    log(f"cytosol reactive: {nreact} A+B->C in {frames} frames, {out['value_incl_reactions']:.3e} p-steps/s incl. "
+        f"reactions; per frame: candidates + copy {out['candidates_and_copy_ms_per_frame']:.1f} ms, host decision "
+        f"{out['host_decision_ms_per_frame']:.1f} ms, device compaction {out['apply_ms_per_frame']:.1f} ms")
+    return out
+
+
 def bench_cytosol(steps=100, warmup=10, rdf_steps=200, frames=10, plain_steps=20, train_steps=3000):
     """Combined workload: 64 replicas (5.06M particles, phi = 0.3, 8 species,
     two wells, A+B candidates) with a 256-disk membrane acting on the two
@@ -389,6 +462,7 @@ def bench_cytosol(steps=100, warmup=10, rdf_steps=200, frames=10, plain_steps=20
         tot += w1
         off += c
     res["plain_exact_per_replica_value"] = n * plain_steps / (tot / 1e3)
+    res["reactive"] = cytosol_reactions(spec, boxes, counts, xyz, types, pieces, mtypes, w, norm)
     res["speedup_batched_surrogate_vs_plain"] = out["surrogate"]["value"] / res["plain_exact_per_replica_value"]
     res["speedup_batched_surrogate_vs_batched_exact"] = out["surrogate"]["value"] / out["exact"]["value"]
     log(f"cytosol: batched exact {out['exact']['value']:.3e}, batched surrogate {out['surrogate']['value']:.3e}, "

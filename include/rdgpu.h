@@ -188,6 +188,11 @@ int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long* count);
  *      on the device (the reference rebuilds every array on the host and
  *      uploads it again, CudaSimulation.cu:936-970). ---- */
 int rd_apply_reactions(rd_system* h, const float* creactions);
+/* The same in a handle of replicas (no counterpart in the reference, which
+ * holds one system): entry r that only creates (index == -1) puts its particle
+ * into replica creation_replica[r]; a type change stays in the replica of its
+ * index.  Replica-local order: survivors, then new particles in entry order. */
+int rd_apply_reactions_replicas(rd_system* h, const float* creactions, const int* creation_replica);
 
 /* ---- output files of the reference's stand-alone driver
  *      (outputHandler.cpp:17-148, opened in lib/ReaDDyGPU_Cpp/ReaDDyGPU.cpp:68-85):

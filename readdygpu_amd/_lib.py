@@ -121,6 +121,7 @@ _SIGS = {
     "rd_traj_write_frame": (C.c_int, [P, C.c_int, C.c_int, P, P]),
     "rd_traj_close": (C.c_int, [P]),
     "rd_rdf_write": (C.c_int, [C.c_char_p, C.c_int, P, C.c_int, C.c_double, P]),
+    "rd_apply_reactions_replicas": (C.c_int, [P, P, P]),
     "rd_simulate_async": (C.c_int, [P, C.c_int]),
     "rd_frame_copy": (C.c_int, [P]),
     "rd_frame_wait": (C.c_int, [P, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]),

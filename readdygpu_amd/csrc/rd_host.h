@@ -43,7 +43,7 @@ struct rd_system {
     std::vector<Rep> reps;
     std::vector<RdRep> rep_dev;
     std::vector<RdWave> wave_tab;
-    std::vector<int> hrep_of;
+    std::vector<long> tcount;        // particles per (replica, type), [R * nt]
     int nwaves = 0;
 
     // particles in index order (host mirror)

@@ -232,6 +232,16 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__
     if (slot_of) slot_of[gid_n[s]] = dst;
 }
 
+// replica of each particle index (replica r owns indices [off_r, off_r + n_r),
+// the same range as its slots)
+__global__ __launch_bounds__(RD_BLOCK) void k_rep_of(const RdWave* __restrict__ waves, int nwaves,
+                                                     int* __restrict__ rep_of)
+{
+    const RdWave it = wave_item(waves, nwaves, wave_id());
+    const int s = it.s0 + (threadIdx.x & 63);
+    if (s < it.s1) rep_of[s] = it.rep;
+}
+
 // positions back to particle-index order
 __global__ __launch_bounds__(RD_BLOCK) void k_unsort(const float4* __restrict__ ps, const RdRep* __restrict__ reps,
                                                      const RdWave* __restrict__ waves, int nwaves,
@@ -249,11 +259,14 @@ __global__ __launch_bounds__(RD_BLOCK) void k_unsort(const float4* __restrict__ 
 // Reaction compaction (the cReactions protocol, lib/ReaDDyGPU.cpp:303-463,
 // on the device): survivors keep their order, so the new index of particle i
 // is i minus the number of removed indices below it (binary search in the
-// sorted removal list); removed particles are dropped.
+// sorted removal list) plus the particles created in earlier replicas;
+// removed particles are dropped.
 __global__ __launch_bounds__(RD_BLOCK) void k_react_compact(const float4* __restrict__ ps,
                                                             const RdRep* __restrict__ reps,
                                                             const RdWave* __restrict__ waves, int nwaves,
                                                             const int* __restrict__ del, int ndel,
+                                                            const int* __restrict__ shift, int nt,
+                                                            uint32_t* __restrict__ delcnt,
                                                             float* __restrict__ xyz, int* __restrict__ type)
 {
     const RdWave it = wave_item(waves, nwaves, wave_id());
@@ -267,10 +280,26 @@ __global__ __launch_bounds__(RD_BLOCK) void k_react_compact(const float4* __rest
         if (del[mid] < gi) lo = mid + 1;
         else hi = mid;
     }
-    if (lo < ndel && del[lo] == gi) return;
-    const size_t ni = (size_t)(gi - lo);
+    if (lo < ndel && del[lo] == gi) {
+        atomicAdd(&delcnt[it.rep * nt + w_type(p.w)], 1u);       // removed particles per (replica, type)
+        return;
+    }
+    const size_t ni = (size_t)(gi - lo + shift[it.rep]);
     xyz[3 * ni] = p.x; xyz[3 * ni + 1] = p.y; xyz[3 * ni + 2] = p.z;
     type[ni] = w_type(p.w);
+}
+
+// new particles of a reaction list to their places (after the survivors of
+// their replica)
+__global__ __launch_bounds__(RD_BLOCK) void k_react_place(const float* __restrict__ bxyz, const int* __restrict__ bty,
+                                                          const int* __restrict__ bdst, int nb,
+                                                          float* __restrict__ xyz, int* __restrict__ type)
+{
+    const int k = blockIdx.x * RD_BLOCK + threadIdx.x;
+    if (k >= nb) return;
+    const size_t d = (size_t)bdst[k];
+    xyz[3 * d] = bxyz[3 * k]; xyz[3 * d + 1] = bxyz[3 * k + 1]; xyz[3 * d + 2] = bxyz[3 * k + 2];
+    type[d] = bty[k];
 }
 
 // reference cell index (replica-local) per particle
@@ -752,6 +781,8 @@ extern "C" int rd_set_particles(rd_system* h, int n, const float* xyz, const int
     h->n = n;
     h->hxyz.assign(xyz, xyz + 3 * (size_t)n);
     h->htype.assign(type, type + n);
+    h->tcount.assign((size_t)h->nt, 0);
+    for (int i = 0; i < n; ++i) h->tcount[type[i]]++;
     h->particles_dirty = true;
     h->dirty = true;
     return RD_OK;
@@ -788,6 +819,10 @@ extern "C" int rd_set_replicas(rd_system* h, int nrep, const double* bounds, con
     h->n = (int)total;
     h->hxyz.assign(xyz, xyz + 3 * (size_t)total);
     h->htype.assign(type, type + total);
+    h->tcount.assign((size_t)nrep * h->nt, 0);
+    base = 0;
+    for (int r = 0; r < nrep; ++r)
+        for (int i = 0; i < counts[r]; ++i, ++base) h->tcount[(size_t)r * h->nt + type[base]]++;
     h->particles_dirty = true;
     h->dirty = true;
     return RD_OK;
@@ -983,16 +1018,15 @@ static int upload(rd_system* h)
                                 hipMemcpyHostToDevice, h->stream));
     }
     if (h->n > 0) {
-        h->hrep_of.resize(h->n);
-        for (size_t r = 0; r < h->rep_dev.size(); ++r)
-            for (int i = 0; i < h->rep_dev[r].n; ++i) h->hrep_of[h->rep_dev[r].off + i] = (int)r;
+        // replica of each particle index: the wave table covers the same ranges
+        hipLaunchKernelGGL(k_rep_of, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_waves, h->nwaves,
+                           h->d_rep_of);
         if (!h->io_on_device) {
             HIPCK(h, hipMemcpyAsync(h->d_xyz_io, h->hxyz.data(), sizeof(float) * 3 * h->n, hipMemcpyHostToDevice,
                                     h->stream));
             HIPCK(h, hipMemcpyAsync(h->d_type_io, h->htype.data(), sizeof(int) * h->n, hipMemcpyHostToDevice,
                                     h->stream));
         }
-        HIPCK(h, hipMemcpyAsync(h->d_rep_of, h->hrep_of.data(), sizeof(int) * h->n, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io, h->d_type_io,
                            h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_gid);
     }
@@ -1285,9 +1319,7 @@ extern "C" int rd_rdf_calculate(rd_system* h)
     // per-frame normalisation constants (particle numbers may change by reactions)
     const int nt = h->nt;
     for (int r = 0; r < h->rdf_nrep; ++r) {
-        std::vector<long> npt(nt, 0);
-        const RdRep& R = h->rep_dev[r];
-        for (int i = 0; i < R.n; ++i) npt[h->htype[R.off + i]]++;
+        const long* npt = &h->tcount[(size_t)r * nt];
         for (int a = 0; a < nt; ++a)
             for (int b = 0; b < nt; ++b)
                 h->rdf_pairs_sum[((size_t)r * nt + a) * nt + b] += (double)npt[a] * (double)(npt[b] - (a == b ? 1 : 0));
@@ -1396,22 +1428,32 @@ extern "C" int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long
 // (lib/ReaDDyGPU.cpp:303-365): index != -1 removes that particle, newType != -1
 // appends a particle of that type at (x, y, z); both is a type change (the
 // particle is removed and its successor appended).  Survivors keep their
-// order, new particles follow in entry order.  Applied on the device: the
-// sorted state is compacted into index order, new particles are copied in,
-// and the cell list is rebuilt; no particle data crosses PCIe.
-extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
+// order, new particles follow in entry order, per replica.  Applied on the
+// device: the sorted state is compacted into index order, new particles are
+// placed, and the cell list is rebuilt; no particle data crosses PCIe.
+static int apply_reactions(rd_system* h, const float* cr, const int* creation_replica)
 {
-    if (!h || !cr) return RD_EINVAL;
     if (!h->initialized) return set_err(h, RD_ESTATE, "not initialized");
-    if (h->reps.size() != 1) return set_err(h, RD_ESTATE, "reactions need a single-system handle");
     int nr = (int)cr[0];
     if (nr <= 0) return RD_OK;
     int rc = refresh(h);
     if (rc) return rc;
     const int n = h->n;
+    const int R = (int)h->reps.size();
+    const int nt = h->nt;
+    // replica of a particle index (replicas hold consecutive index ranges)
+    auto rep_of = [&](int i) {
+        int lo = 0, hi = R - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (h->rep_dev[mid].off <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
     std::vector<int> del;
-    std::vector<float> cx;
-    std::vector<int> ct;
+    struct Born { int rep, type; float x, y, z; };
+    std::vector<Born> born;
     for (int r = 0; r < nr; ++r) {
         const float* e = &cr[1 + 6 * r];
         int newType = (int)e[1];
@@ -1422,59 +1464,88 @@ extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
         }
         if (newType != -1) {
             if (newType < 0 || newType >= h->nt) return set_err(h, RD_EINVAL, "reaction creates an unknown type");
-            cx.push_back(e[2]); cx.push_back(e[3]); cx.push_back(e[4]);
-            ct.push_back(newType);
+            int rep = 0;
+            if (index != -1) rep = rep_of(index);
+            else if (R > 1) {
+                if (!creation_replica) return set_err(h, RD_EINVAL, "a new particle needs its replica");
+                rep = creation_replica[r];
+                if (rep < 0 || rep >= R) return set_err(h, RD_EINVAL, "replica index out of range");
+            }
+            born.push_back({ rep, newType, e[2], e[3], e[4] });
         }
     }
     std::sort(del.begin(), del.end());
     if (std::adjacent_find(del.begin(), del.end()) != del.end())
         return set_err(h, RD_EINVAL, "reaction deletes a nonexistent particle");
-    const rd_system::Rep& A = h->reps[0];
-    rc = validate_particles(h, (int)ct.size(), cx.data(), ct.data(), A.lo, A.hi, n - (int)del.size());
-    if (rc) return rc;
-    const int nkeep = n - (int)del.size();
-    const int n_new = nkeep + (int)ct.size();
+    for (const Born& b : born) {
+        float x[3] = { b.x, b.y, b.z };
+        rc = validate_particles(h, 1, x, &b.type, h->reps[b.rep].lo, h->reps[b.rep].hi, 0);
+        if (rc) return rc;
+    }
+    // per replica: removed, created; new offsets; shift of the survivors
+    std::vector<int> ndel(R, 0), ncre(R, 0), shift(R, 0), noff(R + 1, 0);
+    for (int i : del) ndel[rep_of(i)]++;
+    for (const Born& b : born) ncre[b.rep]++;
+    for (int r = 0; r < R; ++r) {
+        shift[r] = (r ? shift[r - 1] + ncre[r - 1] : 0);
+        noff[r + 1] = noff[r] + h->reps[r].n - ndel[r] + ncre[r];
+    }
+    const int n_new = noff[R];
     if (n_new > RD_MAX_PARTICLES) return set_err(h, RD_EINVAL, "too many particles");
     if (n_new > h->cap) {
         rc = grow_particles(h, n_new, true);
         if (rc) return rc;
     }
-    int* d_del = nullptr;
-    if (!del.empty()) {
-        HIPCK(h, hipMalloc(&d_del, sizeof(int) * del.size()));
-        HIPCK(h, hipMemcpyAsync(d_del, del.data(), sizeof(int) * del.size(), hipMemcpyHostToDevice, h->stream));
+    // destinations of the new particles: after the survivors of their replica
+    std::vector<int> fill(R, 0), bdst(born.size()), bty(born.size());
+    std::vector<float> bxyz(3 * born.size());
+    for (size_t k = 0; k < born.size(); ++k) {
+        const Born& b = born[k];
+        bdst[k] = noff[b.rep] + h->reps[b.rep].n - ndel[b.rep] + fill[b.rep]++;
+        bty[k] = b.type;
+        bxyz[3 * k] = b.x; bxyz[3 * k + 1] = b.y; bxyz[3 * k + 2] = b.z;
     }
-    if (n > 0)
+    int *d_del = nullptr, *d_shift = nullptr, *d_bdst = nullptr, *d_bty = nullptr;
+    float* d_bxyz = nullptr;
+    uint32_t* d_delcnt = nullptr;
+    std::vector<uint32_t> delcnt((size_t)R * nt, 0);
+    hipError_t e = hipSuccess;
+    auto up = [&](auto** dst, const auto* src, size_t cnt) {
+        if (e != hipSuccess || cnt == 0) return;
+        e = hipMalloc((void**)dst, sizeof(**dst) * cnt);
+        if (e == hipSuccess) e = hipMemcpyAsync(*dst, src, sizeof(**dst) * cnt, hipMemcpyHostToDevice, h->stream);
+    };
+    up(&d_del, del.data(), del.size());
+    up(&d_shift, shift.data(), (size_t)R);
+    up(&d_bdst, bdst.data(), bdst.size());
+    up(&d_bty, bty.data(), bty.size());
+    up(&d_bxyz, bxyz.data(), bxyz.size());
+    up(&d_delcnt, delcnt.data(), delcnt.size());
+    if (e == hipSuccess && n > 0)
         hipLaunchKernelGGL(k_react_compact, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_reps,
-                           h->d_waves, h->nwaves, d_del, (int)del.size(), h->d_xyz_io, h->d_type_io);
-    if (!ct.empty()) {
-        HIPCK(h, hipMemcpyAsync(h->d_xyz_io + 3 * (size_t)nkeep, cx.data(), sizeof(float) * cx.size(),
-                                hipMemcpyHostToDevice, h->stream));
-        HIPCK(h, hipMemcpyAsync(h->d_type_io + nkeep, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice,
-                                h->stream));
-    }
-    hipError_t ke = hipGetLastError();
+                           h->d_waves, h->nwaves, d_del, (int)del.size(), d_shift, nt, d_delcnt, h->d_xyz_io,
+                           h->d_type_io);
+    if (e == hipSuccess && !born.empty())
+        hipLaunchKernelGGL(k_react_place, dim3(nblk((long)born.size(), RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream,
+                           d_bxyz, d_bty, d_bdst, (int)born.size(), h->d_xyz_io, h->d_type_io);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess && d_delcnt)
+        e = hipMemcpyAsync(delcnt.data(), d_delcnt, sizeof(uint32_t) * delcnt.size(), hipMemcpyDeviceToHost, h->stream);
     hipError_t se = hipStreamSynchronize(h->stream);
-    if (d_del) (void)hipFree(d_del);
-    if (ke != hipSuccess || se != hipSuccess)
-        return set_err(h, RD_EHIP, std::string("reaction compaction: ") + hipGetErrorString(ke != hipSuccess ? ke : se));
-    // host mirrors: types follow the compaction; positions live on the device
+    for (void* p : { (void*)d_del, (void*)d_shift, (void*)d_bdst, (void*)d_bty, (void*)d_bxyz, (void*)d_delcnt })
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess || se != hipSuccess)
+        return set_err(h, RD_EHIP, std::string("reaction compaction: ") + hipGetErrorString(e != hipSuccess ? e : se));
+    // particle counts per (replica, type); the host position/type mirrors are
+    // refreshed from the device when next read
+    for (size_t k = 0; k < delcnt.size(); ++k) h->tcount[k] -= (long)delcnt[k];
+    for (const Born& b : born) h->tcount[(size_t)b.rep * nt + b.type]++;
+    h->hxyz.resize(3 * (size_t)n_new);
+    h->htype.resize((size_t)n_new);
     auto new_index = [&](int i) {
         auto it = std::lower_bound(del.begin(), del.end(), i);
-        return (it != del.end() && *it == i) ? -1 : i - (int)(it - del.begin());
+        return (it != del.end() && *it == i) ? -1 : i - (int)(it - del.begin()) + shift[rep_of(i)];
     };
-    {
-        std::vector<int> nt;
-        nt.reserve(n_new);
-        size_t k = 0;
-        for (int i = 0; i < n; ++i) {
-            if (k < del.size() && del[k] == i) { ++k; continue; }
-            nt.push_back(h->htype[i]);
-        }
-        nt.insert(nt.end(), ct.begin(), ct.end());
-        h->htype.swap(nt);
-        h->hxyz.resize(3 * (size_t)n_new);
-    }
     // bonds follow their particles; a bond of a removed particle is dropped
     if (!h->bond_pot.empty()) {
         std::vector<int> bp, bo;
@@ -1487,13 +1558,25 @@ extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
         h->bond_pairs.swap(bp);
         h->bond_pot.swap(bo);
     }
+    for (int r = 0; r < R; ++r) h->reps[r].n = noff[r + 1] - noff[r];
     h->n = n_new;
-    h->reps[0].n = n_new;
     h->snap_valid = false;
     h->io_on_device = true;
     rc = upload(h);
     h->io_on_device = false;
     return rc;
+}
+
+extern "C" int rd_apply_reactions(rd_system* h, const float* cr)
+{
+    if (!h || !cr) return RD_EINVAL;
+    return apply_reactions(h, cr, nullptr);
+}
+
+extern "C" int rd_apply_reactions_replicas(rd_system* h, const float* cr, const int* creation_replica)
+{
+    if (!h || !cr) return RD_EINVAL;
+    return apply_reactions(h, cr, creation_replica);
 }
 
 // ---- frame hand-off (the frame loop of lib/ReaDDyGPU.cpp:235-301) ----

@@ -273,10 +273,17 @@ class Simulation:
         """Back to the state of the last frame_copy (same noise follows)."""
         self._check(self.lib.rd_frame_rewind(self.h))
 
-    def apply_reactions(self, creactions: np.ndarray):
-        """cReactions protocol (lib/ReaDDyGPU.cpp:303-463)."""
+    def apply_reactions(self, creactions: np.ndarray, creation_replica: Optional[np.ndarray] = None):
+        """cReactions protocol (lib/ReaDDyGPU.cpp:303-463); in a handle of
+        replicas, entry r that only creates goes to replica creation_replica[r]."""
         cr = np.ascontiguousarray(creactions, dtype=np.float32)
-        self._check(self.lib.rd_apply_reactions(self.h, _ptr(cr)))
+        if creation_replica is None:
+            self._check(self.lib.rd_apply_reactions(self.h, _ptr(cr)))
+        else:
+            rr = np.ascontiguousarray(creation_replica, dtype=np.int32)
+            if len(rr) < int(cr[0]):
+                raise ValueError("creation_replica needs one entry per reaction")
+            self._check(self.lib.rd_apply_reactions_replicas(self.h, _ptr(cr), _ptr(rr)))
 
     def set_launch_config(self, kernel: int = 0):
         """Force kernel: 0 automatic, 1 per-particle gather."""

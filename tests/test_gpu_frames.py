@@ -8,6 +8,7 @@ exactly like a fresh handle built from the same particles."""
 import numpy as np
 import pytest
 
+from oracle import oracle as orc
 from readdygpu_amd import RdError, Simulation, workloads
 
 pytestmark = pytest.mark.gpu
@@ -79,7 +80,7 @@ def test_device_reactions_leave_a_complete_state():
     spec, xyz, types = workloads.plain_10k(n=2000)
     rng = np.random.default_rng(3)
     z = rng.normal(size=(4, 3, 2000 - 3 + 5)).astype(np.float32)
-    with Simulation(spec, xyz, types) as sim:
+    with Simulation(spec, xyz, types, rdf=(10.0, 40)) as sim:
         sim.simulate(3)
         e = [8]
         for i in (11, 700, 1999):
@@ -88,6 +89,15 @@ def test_device_reactions_leave_a_complete_state():
             e += [0, k % 2, 10.0 + k, 20.0, 30.0, -1]
         sim.apply_reactions(np.array(e, np.float32))
         x1, t1 = sim.copy_pos_from_device()
+        # the RDF normalisation uses the particle numbers after the reactions
+        sim.call_rdf_calculation()
+        cnt, frames = sim.copy_rdf_matrix()
+        g = sim.normalize_rdf()
+        o = orc.OracleSystem(spec)
+        ref = o.rdf_counts(x1, t1, 10.0, 40, cells=False)
+        assert np.array_equal(cnt, ref)
+        npt = [int(np.sum(t1 == k)) for k in range(spec.n_types)]
+        assert np.allclose(g, orc.rdf_normalize(spec.n_types, 40, 10.0, spec.volume(), npt, 1, ref), rtol=1e-12)
         sim.set_noise_array(z)
         sim.simulate(4)
         xa, ta = sim.copy_pos_from_device()

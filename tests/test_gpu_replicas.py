@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from readdygpu_amd import Simulation, workloads
+from readdygpu_amd import RdError, Simulation, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -146,3 +146,61 @@ def test_full_batched_workload_runs():
         assert np.isfinite(xs).all()
         assert (xs >= lo).all() and (xs <= hi).all(), f"replica {r}"
         off += n
+
+
+def test_replica_reactions_equal_separate_runs(batch):
+    """Reactions in a handle of replicas: removals, creations (placed by
+    creation_replica) and type changes give, replica for replica, the state
+    and the following trajectory of the same reactions applied to each system
+    alone."""
+    spec, boxes, counts, xyz, types = batch
+    R = len(counts)
+    offs = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    local = {}            # replica -> list of local entries (readdyId, newType, x, y, z, index)
+    for r in range(R):
+        b = boxes[r]
+        e = []
+        if r % 3 == 0:
+            e.append([0, -1, 0, 0, 0, 0])                                  # remove local 0
+        if r % 4 == 1:
+            e.append([0, 1, b[0] + 1.0, b[2] + 2.0, b[4] + 3.0, -1])       # create one B
+        if r % 5 == 2:
+            e.append([0, 0, b[0] + 4.0, b[2] + 4.0, b[4] + 4.0, 3])        # type change of local 3
+        if r == R - 1:
+            e.append([0, -1, 0, 0, 0, int(counts[r]) - 1])                  # remove the last particle
+        local[r] = e
+    glob, rep = [], []
+    for r in range(R):
+        for x in local[r]:
+            g = list(x)
+            if g[5] != -1:
+                g[5] += int(offs[r])
+            glob.append(g)
+            rep.append(r)
+    cr = np.array([len(glob)] + [v for g in glob for v in g], np.float32)
+    with Simulation(spec, xyz, types, replicas=(boxes, counts)) as sim:
+        sim.simulate(5)
+        sim.apply_reactions(cr, creation_replica=np.array(rep, np.int32))
+        info = sim.info()
+        x1, t1 = sim.copy_pos_from_device()
+        sim.simulate(10)
+        x2, t2 = sim.copy_pos_from_device()
+    new_counts = []
+    for r in range(R):
+        s, x, t, _ = replica(spec, boxes, counts, xyz, types, r)
+        with Simulation(s, x, t) as one:
+            one.simulate(5)
+            if local[r]:
+                one.apply_reactions(np.array([len(local[r])] + [v for g in local[r] for v in g], np.float32))
+            xa, ta = one.copy_pos_from_device()
+            one.simulate(10)
+            xb, tb = one.copy_pos_from_device()
+        o = int(sum(new_counts))
+        n = len(ta)
+        new_counts.append(n)
+        assert np.array_equal(x1[o:o + n], xa) and np.array_equal(t1[o:o + n], ta), f"replica {r}"
+        assert np.array_equal(x2[o:o + n], xb) and np.array_equal(t2[o:o + n], tb), f"replica {r}"
+    assert info.n == sum(new_counts) and info.n_replicas == R
+    with Simulation(spec, xyz, types, replicas=(boxes, counts)) as sim:
+        with pytest.raises(RdError):             # a pure creation needs its replica
+            sim.apply_reactions(np.array([1, 0, 1, 1.0, 1.0, 1.0, -1], np.float32))
