@@ -327,13 +327,16 @@ typedef struct {
                               testing distances first and evaluating the listed hits after.
                               AUTO = GATHER */
     int used_kernel;       /* out: kernel in use */
+    int block;             /* threads per workgroup of the force kernel: 64, 128, 256 or 512; 0 = 256 */
+    int used_block;        /* out: workgroup size in use */
 } rd_launch_config;
 int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg);
 int rd_get_launch_config(rd_system* h, rd_launch_config* cfg);
 
 /* Autotuner: times `trial_steps` steps for each candidate cell size (factors
- * of the interaction range) and force kernel, keeps the fastest, and restores
- * the particle state and step counter, so tuning does not change the run. */
+ * of the interaction range) and force-kernel workgroup size, keeps the
+ * fastest, and restores the particle state and step counter, so tuning does
+ * not change the run. */
 int rd_autotune(rd_system* h, int trial_steps, double* best_ms_per_step);
 
 #ifdef __cplusplus

@@ -56,6 +56,8 @@ class rd_launch_config(C.Structure):
     _fields_ = [
         ("kernel", C.c_int),
         ("used_kernel", C.c_int),
+        ("block", C.c_int),
+        ("used_block", C.c_int),
     ]
 
 

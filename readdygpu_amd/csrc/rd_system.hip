@@ -42,7 +42,10 @@ __device__ __forceinline__ int wave_id() { return xcd_block(blockIdx.x, gridDim.
 // One BD step for the particle in slot s with the per-particle global gather
 // (CudaSimulation.cu:2314-2465 fused): gradient, Euler-Maruyama
 // (CudaSimulation.cu:1852), periodic wrap (:1857-1858), next cell key.
-__global__ __launch_bounds__(RD_BLOCK) void k_force_update(const float4* __restrict__ ps, float4* __restrict__ pn,
+// BLOCK threads per workgroup (the autotuner's choice); the type-pair table is
+// staged in LDS once per workgroup.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict__ ps, float4* __restrict__ pn,
                                                            uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                                                            uint32_t* __restrict__ count,
                                                            const uint32_t* __restrict__ start,
@@ -54,9 +57,10 @@ __global__ __launch_bounds__(RD_BLOCK) void k_force_update(const float4* __restr
                                                            const float* __restrict__ noise, int ntot)
 {
     __shared__ RdTab T;
-    __shared__ RdHitList il[RD_BLOCK / 64];
+    __shared__ RdHitList il[BLOCK / 64];
     load_tab(T, P);
-    const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(wave_id()));
+    const int w = xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+    const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w));
     const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
     if (s >= (uint32_t)it.s1) return;
     const RdRep R = reps[it.rep];
@@ -916,12 +920,23 @@ static void launch_sort(rd_system* h, int advance)
 
 static int wave_blocks(const rd_system* h) { return std::max(1, nblk(h->nwaves, RD_BLOCK / 64)); }
 
+template <int B>
+static void launch_force(rd_system* h)
+{
+    hipLaunchKernelGGL(k_force_update<B>, dim3(std::max(1, nblk(h->nwaves, B / 64))), dim3(B), 0, h->stream, h->d_ps,
+                       h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves,
+                       h->d_pot1, h->d_st, h->d_noise, h->n);
+}
+
 void rd_host_launch_force(rd_system* h)
 {
     if (h->n <= 0) return;
-    hipLaunchKernelGGL(k_force_update, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_pn, h->d_key,
-                       h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st,
-                       h->d_noise, h->n);
+    switch (h->block) {
+    case 64: launch_force<64>(h); break;
+    case 128: launch_force<128>(h); break;
+    case 512: launch_force<512>(h); break;
+    default: launch_force<256>(h); break;
+    }
 }
 
 static void launch_step(rd_system* h)
@@ -996,6 +1011,7 @@ static int upload(rd_system* h)
     h->P.gext = h->ext_active() ? h->d_gext : nullptr;
     h->P.ext_mask = h->comp_mask;
     h->kern = RD_KERNEL_GATHER;
+    h->block = (h->lc.block == 64 || h->lc.block == 128 || h->lc.block == 512) ? h->lc.block : 256;
     HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
     HIPCK(h, hipMemcpyAsync(h->d_reps, h->rep_dev.data(), sizeof(RdRep) * h->rep_dev.size(), hipMemcpyHostToDevice,
                             h->stream));
@@ -1148,6 +1164,8 @@ extern "C" int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg)
 {
     if (!h || !cfg) return RD_EINVAL;
     if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_GATHER) return set_err(h, RD_EINVAL, "bad kernel");
+    if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256 && cfg->block != 512)
+        return set_err(h, RD_EINVAL, "workgroup size must be 64, 128, 256 or 512");
     h->lc = *cfg;
     h->dirty = true;
     return RD_OK;
@@ -1158,6 +1176,7 @@ extern "C" int rd_get_launch_config(rd_system* h, rd_launch_config* cfg)
     if (!h || !cfg) return RD_EINVAL;
     *cfg = h->lc;
     cfg->used_kernel = h->kern;
+    cfg->used_block = h->block;
     return RD_OK;
 }
 

@@ -50,8 +50,11 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     if (user_cell > 0 || base <= 0) cells.push_back(user_cell > 0 ? user_cell : h->cell);
     else
         for (double f : { 1.01, 1.2, 1.45, 1.75, 2.1 }) cells.push_back(f * base);
-    struct Cfg { int kernel; };
-    const Cfg cfgs[] = { { RD_KERNEL_GATHER } };
+    // workgroup sizes of the force kernel (a size the caller fixed is kept)
+    struct Cfg { int kernel, block; };
+    std::vector<Cfg> cfgs;
+    for (int b : { 64, 128, 256, 512 })
+        if (user_lc.block == 0 || user_lc.block == b) cfgs.push_back({ RD_KERNEL_GATHER, b });
     double best = 1e30, best_cell = cells[0];
     Cfg best_cfg = cfgs[0];
     for (double c : cells) {
@@ -59,6 +62,7 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
             h->prm.cell_size = c;
             rd_launch_config lc{};
             lc.kernel = k.kernel;
+            lc.block = k.block;
             rc = rd_set_launch_config(h, &lc);
             if (rc == RD_OK) rc = rd_host_set_state(h, sx, st);
             if (rc == RD_OK) rc = rd_host_refresh(h);
@@ -74,6 +78,7 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     h->prm.cell_size = user_cell > 0 ? user_cell : best_cell;
     rd_launch_config lc = user_lc;
     lc.kernel = best_cfg.kernel;
+    lc.block = best_cfg.block;
     rc = rd_set_launch_config(h, &lc);
     if (rc == RD_OK) rc = rd_host_set_state(h, sx, st);
     if (rc == RD_OK) rc = rd_host_refresh(h);

@@ -285,10 +285,12 @@ class Simulation:
                 raise ValueError("creation_replica needs one entry per reaction")
             self._check(self.lib.rd_apply_reactions_replicas(self.h, _ptr(cr), _ptr(rr)))
 
-    def set_launch_config(self, kernel: int = 0):
-        """Force kernel: 0 automatic, 1 per-particle gather."""
+    def set_launch_config(self, kernel: int = 0, block: int = 0):
+        """Force kernel (0 automatic, 1 per-particle gather) and its workgroup
+        size (64, 128, 256, 512; 0 = 256)."""
         c = _lib.rd_launch_config()
         c.kernel = int(kernel)
+        c.block = int(block)
         self._check(self.lib.rd_set_launch_config(self.h, C.byref(c)))
 
     def launch_config(self) -> dict:

@@ -167,7 +167,7 @@ def test_launch_configs_give_identical_trajectories(which, c1):
         spec, xyz, types = workloads.plain_1m(n=200_000, box=585.0)
     ref = None
     rc = float(spec.pair_range().max())
-    shapes = [dict(kernel=1), dict(kernel=0)]
+    shapes = [dict(kernel=1), dict(kernel=0, block=64), dict(kernel=1, block=512), dict(kernel=1, block=128)]
     for cell in (0.0, 1.3 * rc, 2.1 * rc):
         for shape in shapes:
             spec.cell_size = cell
@@ -177,7 +177,7 @@ def test_launch_configs_give_identical_trajectories(which, c1):
                 x, _ = sim.copy_pos_from_device()
                 g, _ = sim.gradients()
                 used = sim.launch_config()
-            assert used["used_kernel"] == 1, used
+            assert used["used_kernel"] == 1 and used["used_block"] == (shape.get("block") or 256), used
             if ref is None:
                 ref = (x, g)
             assert np.array_equal(x, ref[0]), (cell, shape)
