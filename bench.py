@@ -239,6 +239,37 @@ def bench_membrane(steps=200, frames=20):
             "passed": bool(rmse <= 0.01 and dev <= 0.005)}
 
 
+def bench_passes(sim, reps=10):
+    """The per-invocation passes beside the step on the same state: RDF
+    (0-20 nm, 200 bins, all type pairs), A+B reaction candidates at 4 nm,
+    and the neighbour-pair emission."""
+    sim.configure_rdf(20.0, 200)
+    sim.call_rdf_calculation()
+    sim.copy_rdf_matrix()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.call_rdf_calculation()
+    sim.copy_rdf_matrix()
+    t_rdf = (time.perf_counter() - t0) / reps
+    c = sim.reaction_candidates()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c = sim.reaction_candidates()
+    t_cand = (time.perf_counter() - t0) / reps
+    p = sim.neighbor_pairs()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p = sim.neighbor_pairs()
+    t_pairs = (time.perf_counter() - t0) / reps
+    res = {"rdf_ms": t_rdf * 1e3, "rdf": "0-20 nm, 200 bins, 4x4 type pairs",
+           "candidates_ms": t_cand * 1e3, "candidates": int(len(c)),
+           "neighbor_pairs_ms": t_pairs * 1e3, "neighbor_pairs": int(len(p)),
+           "note": "host wall time per call, including the copy of the pair lists to the host"}
+    log(f"passes: RDF {res['rdf_ms']:.2f} ms, candidates {res['candidates_ms']:.2f} ms ({len(c)}), "
+        f"neighbour pairs {res['neighbor_pairs_ms']:.2f} ms ({len(p)})")
+    return res
+
+
 def bench_frames(frames=20, spf=100):
     """Frame hand-off on the 1M workload: the sequential loop of the
     reference (simulate, synchronous copy, host packs JPos = [N, (x,y,z,
@@ -549,6 +580,7 @@ def main():
         "bytes_alg_per_launch": B_ALG * n,
         "sort_avg_us": sort_ms / args.steps * 1e3,
     }
+    passes = bench_passes(sim)
     sim.close()
     cpu = None
     if not args.no_cpu:
@@ -585,6 +617,7 @@ def main():
             "force_kernel": {1: "gather"}.get(lc["used_kernel"], "?"),
         },
         "roofline": roofline,
+        "passes": passes,
         "cpu_baseline": cpu,
         "extras": extras,
     }

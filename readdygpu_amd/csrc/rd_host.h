@@ -119,7 +119,6 @@ struct rd_system {
 
     // pair emission
     uint2* d_pairs = nullptr;
-    unsigned long long* d_pcount = nullptr;
     size_t pairs_cap = 0;
 
     // captured steps: one graph per step count

@@ -399,11 +399,17 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf(const float4* __restrict__ ps,
 }
 
 // pairs (i<j by index) with d2 < thr2[ti][tj]; global particle indices
+// Pair sets (neighbour pairs d^2 < rc^2, reaction candidates d^2 < R^2) in
+// global-index order without a global sort: a counting pass writes the number
+// of partners j > i of every particle i, an exclusive scan turns the counts
+// into offsets, and an emitting pass writes each particle's pairs into its own
+// segment and sorts that (short) segment by j.
+template <bool EMIT>
 __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ ps, const uint32_t* __restrict__ start,
                                                     const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
                                                     const RdWave* __restrict__ waves, int nwaves,
-                                                    const float* __restrict__ thr2, int S, uint2* __restrict__ out,
-                                                    unsigned long long cap, unsigned long long* __restrict__ counter)
+                                                    const float* __restrict__ thr2, int S,
+                                                    uint32_t* __restrict__ cnt_off, uint2* __restrict__ out)
 {
     const RdWave it = wave_item(waves, nwaves, wave_id());
     const int s = it.s0 + (threadIdx.x & 63);
@@ -413,8 +419,11 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ p
     const int nt = P->nt;
     int ti = w_type(p.w);
     uint32_t ii = w_index(p.w);
+    const uint32_t gi = (uint32_t)R.off + ii;
     int cx, cy, cz;
     cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
+    const uint32_t o = EMIT ? cnt_off[gi] : 0u;
+    uint32_t c = 0;
     for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
         float4 q = ps[j];
         uint32_t jj = w_index(q.w);
@@ -426,10 +435,20 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ p
         d2 = d2 + dy * dy;
         d2 = d2 + dz * dz;
         if (d2 < thr2[ti * nt + w_type(q.w)]) {
-            unsigned long long slot = atomicAdd(counter, 1ull);
-            if (slot < cap) out[slot] = make_uint2((uint32_t)R.off + ii, (uint32_t)R.off + jj);
+            if (EMIT) {
+                // insertion into the sorted segment
+                const uint32_t gj = (uint32_t)R.off + jj;
+                uint32_t k = c;
+                while (k > 0 && out[o + k - 1].y > gj) {
+                    out[o + k] = out[o + k - 1];
+                    --k;
+                }
+                out[o + k] = make_uint2(gi, gj);
+            }
+            ++c;
         }
     });
+    if (!EMIT) cnt_off[gi] = c;
 }
 
 // ===========================================================================
@@ -489,14 +508,14 @@ static void free_dev(rd_system* h)
     free_bonds(h);
     free_frames(h);
     void* ptrs[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_bsum,
-                     h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_pairs, h->d_pcount,
+                     h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_pairs,
                      h->d_xyz_io, h->d_type_io, h->d_etot, h->d_reps, h->d_waves, h->d_rep_of };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->d_ps = h->d_pn = nullptr;
     h->d_key = h->d_rank = h->d_count = h->d_start = h->d_bsum = nullptr;
     h->d_P = nullptr; h->d_pot1 = nullptr; h->d_st = nullptr; h->d_noise = nullptr;
-    h->d_rdf = nullptr; h->d_pairs = nullptr; h->d_pcount = nullptr;
+    h->d_rdf = nullptr; h->d_pairs = nullptr;
     h->d_xyz_io = nullptr; h->d_type_io = nullptr; h->d_etot = nullptr;
     h->d_reps = nullptr; h->d_waves = nullptr; h->d_rep_of = nullptr;
     h->cap = 0; h->ncell_alloc = 0; h->pairs_cap = 0; h->noise_cap = 0; h->nrep_alloc = 0; h->nwave_alloc = 0;
@@ -1395,41 +1414,46 @@ static int pairs_common(rd_system* h, bool reactions, int64_t* out, long cap, lo
     for (int a = 0; a < h->nt * h->nt; ++a) m2 = std::max(m2, table[a]);
     if (h->n == 0 || m2 <= 0.0f) { if (count) *count = 0; return RD_OK; }
     int S = std::max(1, (int)std::ceil(std::sqrt((double)m2) * 1.0001 / h->cell));
-    if (!h->d_pcount) HIPCK(h, hipMalloc(&h->d_pcount, sizeof(unsigned long long)));
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        if (h->pairs_cap == 0) {
-            if (h->d_pairs) (void)hipFree(h->d_pairs);
-            h->d_pairs = nullptr;
-            size_t c0 = std::max<size_t>(1 << 16, (size_t)h->n * 4);
-            HIPCK(h, hipMalloc(&h->d_pairs, sizeof(uint2) * c0));
-            h->pairs_cap = c0;
-        }
-        HIPCK(h, hipMemsetAsync(h->d_pcount, 0, sizeof(unsigned long long), h->stream));
-        hipLaunchKernelGGL(k_pairs, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start, h->d_P,
-                           h->d_reps, h->d_waves, h->nwaves, dtable, S, h->d_pairs, (unsigned long long)h->pairs_cap,
-                           h->d_pcount);
-        HIPCK(h, hipGetLastError());
-        unsigned long long total = 0;
-        HIPCK(h, hipMemcpyAsync(&total, h->d_pcount, sizeof(total), hipMemcpyDeviceToHost, h->stream));
-        HIPCK(h, hipStreamSynchronize(h->stream));
-        if (total > h->pairs_cap) {
-            (void)hipFree(h->d_pairs);
-            h->d_pairs = nullptr;
-            HIPCK(h, hipMalloc(&h->d_pairs, sizeof(uint2) * total));
-            h->pairs_cap = total;
-            continue;
-        }
-        std::vector<uint64_t> v(total);
-        std::vector<uint2> raw(total);
-        if (total) HIPCK(h, hipMemcpy(raw.data(), h->d_pairs, sizeof(uint2) * total, hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < total; ++k) v[k] = ((uint64_t)raw[k].x << 32) | raw[k].y;
-        std::sort(v.begin(), v.end());
-        long nout = (long)std::min<uint64_t>(total, (uint64_t)cap);
-        for (long k = 0; k < nout; ++k) { out[2 * k] = (int64_t)(v[k] >> 32); out[2 * k + 1] = (int64_t)(v[k] & 0xffffffffu); }
-        if (count) *count = (long)total;
-        return RD_OK;
+    // per-particle counts -> offsets (the counting-sort scan over n entries)
+    const int n = h->n;
+    const int nsc = nblk(n, RD_SCAN_TILE);
+    uint32_t *d_cnt = nullptr, *d_off = nullptr, *d_bs = nullptr;
+    hipError_t e = hipMalloc(&d_cnt, sizeof(uint32_t) * ((size_t)n + 16));
+    if (e == hipSuccess) e = hipMalloc(&d_off, sizeof(uint32_t) * ((size_t)n + 16));
+    if (e == hipSuccess) e = hipMalloc(&d_bs, sizeof(uint32_t) * ((size_t)nsc + 16));
+    uint32_t total = 0;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_pairs<false>, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
+                           h->d_P, h->d_reps, h->d_waves, h->nwaves, dtable, S, d_cnt, (uint2*)nullptr);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nsc), dim3(RD_BLOCK), 0, h->stream, d_cnt, n, d_bs);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nsc), dim3(RD_BLOCK), 0, h->stream, d_cnt, n, d_bs, d_off);
+        e = hipGetLastError();
     }
-    return set_err(h, RD_ENOMEM, "pair buffer");
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_off + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && total > h->pairs_cap) {
+        if (h->d_pairs) (void)hipFree(h->d_pairs);
+        h->d_pairs = nullptr;
+        h->pairs_cap = 0;
+        e = hipMalloc(&h->d_pairs, sizeof(uint2) * total);
+        if (e == hipSuccess) h->pairs_cap = total;
+    }
+    const long nout = std::min<long>((long)total, cap);
+    std::vector<uint2> raw((size_t)nout);
+    if (e == hipSuccess && total > 0) {
+        hipLaunchKernelGGL(k_pairs<true>, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
+                           h->d_P, h->d_reps, h->d_waves, h->nwaves, dtable, S, d_off, h->d_pairs);
+        e = hipGetLastError();
+        if (e == hipSuccess && nout > 0)
+            e = hipMemcpyAsync(raw.data(), h->d_pairs, sizeof(uint2) * (size_t)nout, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    for (void* p : { (void*)d_cnt, (void*)d_off, (void*)d_bs })
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return set_err(h, RD_EHIP, std::string("pair emission: ") + hipGetErrorString(e));
+    for (long k = 0; k < nout; ++k) { out[2 * k] = (int64_t)raw[k].x; out[2 * k + 1] = (int64_t)raw[k].y; }
+    if (count) *count = (long)total;
+    return RD_OK;
 }
 
 extern "C" int rd_neighbor_pairs(rd_system* h, int64_t* out, long cap, long* count)

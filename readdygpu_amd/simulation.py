@@ -234,10 +234,19 @@ class Simulation:
         return g
 
     def _pairs(self, fn) -> np.ndarray:
+        # one call with a buffer sized from the previous result; a second one
+        # only when the set outgrew it
+        key = getattr(fn, "__name__", str(id(fn)))
+        cap = max(4096, int(1.25 * getattr(self, "_pair_sizes", {}).get(key, 0)))
         cnt = C.c_long()
-        self._check(fn(self.h, None, 0, C.byref(cnt)))
-        out = np.empty((max(cnt.value, 1), 2), dtype=np.int64)
-        self._check(fn(self.h, _ptr(out), cnt.value, C.byref(cnt)))
+        out = np.empty((cap, 2), dtype=np.int64)
+        self._check(fn(self.h, _ptr(out), cap, C.byref(cnt)))
+        if cnt.value > cap:
+            out = np.empty((cnt.value, 2), dtype=np.int64)
+            self._check(fn(self.h, _ptr(out), cnt.value, C.byref(cnt)))
+        if not hasattr(self, "_pair_sizes"):
+            self._pair_sizes = {}
+        self._pair_sizes[key] = cnt.value
         return out[:cnt.value]
 
     def neighbor_pairs(self) -> np.ndarray:
