@@ -548,10 +548,14 @@ def main():
     sim = Simulation(spec, xyz, types)
     if args.kernel >= 0:
         sim.set_launch_config(kernel=args.kernel)
+    # warm up first: the confinement pulls the particles outside the sphere
+    # onto its surface within the first steps, so the tuned cell size has to
+    # be chosen on the shell that the timed steps see (the autotuner restores
+    # the state it starts from)
+    sim.simulate(args.warmup)
     if not args.no_tune and args.kernel < 0 and args.cell == 0.0:
         ms = sim.autotune(trial_steps=10)
         log(f"autotuned: {ms * 1e3:.1f} us/step")
-    sim.simulate(args.warmup)
     info = sim.info()
     lc = sim.launch_config()
     log(f"lattice {list(info.lattice)} cell {info.cell_size:.3f} nm; kernel {lc}; after {args.warmup} warmup steps")

@@ -14,7 +14,7 @@
 #include "rd_device.h"
 
 #define RD_BLOCK 256
-#define RD_SCAN_ITEMS 16
+#define RD_SCAN_ITEMS 8
 #define RD_SCAN_TILE (RD_BLOCK * RD_SCAN_ITEMS)
 
 struct RdStepState {
