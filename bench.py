@@ -512,6 +512,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
     ap.add_argument("--kernel", type=int, default=-1, help="force kernel: 1 gather")
+    ap.add_argument("--block", type=int, default=0, help="force-kernel workgroup size (64-512; 0: autotuned)")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned workloads")
     ap.add_argument("--only", choices=["batched_replicas", "learned_slab", "learned_membrane", "crowded_cytosol",
                                        "frame_handoff"],
@@ -546,14 +547,14 @@ def main():
     n = len(types)
     log(f"setting up plain_1m n={n}")
     sim = Simulation(spec, xyz, types)
-    if args.kernel >= 0:
-        sim.set_launch_config(kernel=args.kernel)
+    if args.kernel >= 0 or args.block:
+        sim.set_launch_config(kernel=max(args.kernel, 0), block=args.block)
     # warm up first: the confinement pulls the particles outside the sphere
     # onto its surface within the first steps, so the tuned cell size has to
     # be chosen on the shell that the timed steps see (the autotuner restores
     # the state it starts from)
     sim.simulate(args.warmup)
-    if not args.no_tune and args.kernel < 0 and args.cell == 0.0:
+    if not args.no_tune and args.kernel < 0 and args.cell == 0.0 and not args.block:
         ms = sim.autotune(trial_steps=10)
         log(f"autotuned: {ms * 1e3:.1f} us/step")
     info = sim.info()
