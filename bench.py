@@ -555,7 +555,7 @@ def main():
     # the state it starts from)
     sim.simulate(args.warmup)
     if not args.no_tune and args.kernel < 0 and args.cell == 0.0 and not args.block:
-        ms = sim.autotune(trial_steps=10)
+        ms = sim.autotune(trial_steps=20)
         log(f"autotuned: {ms * 1e3:.1f} us/step")
     info = sim.info()
     lc = sim.launch_config()
