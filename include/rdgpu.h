@@ -229,6 +229,20 @@ int rd_frame_rewind(rd_system* h);
 
 int rd_get_info(rd_system* h, rd_info* info);
 
+/* ---- checkpoint / resume.  No counterpart in the reference: its de-facto
+ *      restart is the XML trajectory (outputHandler.cpp:135-148, read by
+ *      CoordReader.cpp:27-48), which drops the step counter, so the noise of a
+ *      restarted run does not continue the original one. ----
+ * rd_checkpoint_write: binary snapshot of the particle state (index order),
+ *   replica boxes, streams and counts, bonds, seed and step counter.
+ * rd_checkpoint_read: load it into a handle built with the same particle types,
+ *   potentials, bond potentials and seed; the run then continues bit for bit
+ *   as the uninterrupted one.
+ * rd_set_step: set the step (noise) counter. */
+int rd_checkpoint_write(rd_system* h, const char* path);
+int rd_checkpoint_read(rd_system* h, const char* path);
+int rd_set_step(rd_system* h, uint64_t step);
+
 /* ---- composite external potential and its learned surrogate ----
  * No counterpart in the reference.  The composite is a geometry of order-one
  * pieces: attractive disks and cylinders (the forms of orderOne,

@@ -125,6 +125,9 @@ _SIGS = {
     "rd_rdf_write": (C.c_int, [C.c_char_p, C.c_int, P, C.c_int, C.c_double, P]),
     "rd_apply_reactions_replicas": (C.c_int, [P, P, P]),
     "rd_simulate_async": (C.c_int, [P, C.c_int]),
+    "rd_checkpoint_write": (C.c_int, [P, C.c_char_p]),
+    "rd_checkpoint_read": (C.c_int, [P, C.c_char_p]),
+    "rd_set_step": (C.c_int, [P, C.c_uint64]),
     "rd_frame_copy": (C.c_int, [P]),
     "rd_frame_wait": (C.c_int, [P, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]),
     "rd_frame_rewind": (C.c_int, [P]),

@@ -806,6 +806,7 @@ extern "C" int rd_set_particles(rd_system* h, int n, const float* xyz, const int
     h->htype.assign(type, type + n);
     h->tcount.assign((size_t)h->nt, 0);
     for (int i = 0; i < n; ++i) h->tcount[type[i]]++;
+    h->snap_valid = false;
     h->particles_dirty = true;
     h->dirty = true;
     return RD_OK;
@@ -846,6 +847,7 @@ extern "C" int rd_set_replicas(rd_system* h, int nrep, const double* bounds, con
     base = 0;
     for (int r = 0; r < nrep; ++r)
         for (int i = 0; i < counts[r]; ++i, ++base) h->tcount[(size_t)r * h->nt + type[base]]++;
+    h->snap_valid = false;
     h->particles_dirty = true;
     h->dirty = true;
     return RD_OK;
@@ -1729,4 +1731,116 @@ extern "C" int rd_get_info(rd_system* h, rd_info* info)
     info->n_replicas = (int)h->reps.size();
     info->n_cells = h->ncell;
     return RD_OK;
+}
+
+// ---- checkpoint / resume ----
+// No counterpart in the reference, whose de-facto restart is the XML
+// trajectory (outputHandler.cpp:135-148, read back by CoordReader.cpp:27-48):
+// that format drops the step counter, so the noise after a restart would not
+// continue the run.  The snapshot holds the particle state in index order, the
+// replica layout, the bonds, the seed and the step (noise) counter.
+static const char RD_CKPT_MAGIC[8] = { 'R', 'D', 'C', 'K', 'P', 'T', '0', '1' };
+
+extern "C" int rd_set_step(rd_system* h, uint64_t step)
+{
+    if (!h) return RD_EINVAL;
+    if (!h->d_st) {                  // not initialized yet: used when the state is created
+        h->step = step;
+        return RD_OK;
+    }
+    HIPCK(h, hipSetDevice(h->dev));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    return rd_host_set_step(h, step);
+}
+
+extern "C" int rd_checkpoint_write(rd_system* h, const char* path)
+{
+    if (!h || !path) return RD_EINVAL;
+    int rc = rd_copy_pos_from_device(h, nullptr, nullptr);
+    if (rc) return rc;
+    FILE* f = fopen(path, "wb");
+    if (!f) return set_err(h, RD_EINVAL, std::string("cannot write ") + path);
+    const uint32_t version = 1, nt = (uint32_t)h->nt, nrep = (uint32_t)h->reps.size(), n = (uint32_t)h->n;
+    const uint32_t nb = (uint32_t)h->bond_pot.size();
+    const uint64_t seed = h->prm.seed, step = h->step;
+    bool ok = fwrite(RD_CKPT_MAGIC, 1, 8, f) == 8;
+    ok = ok && fwrite(&version, 4, 1, f) == 1 && fwrite(&seed, 8, 1, f) == 1 && fwrite(&step, 8, 1, f) == 1;
+    ok = ok && fwrite(&nt, 4, 1, f) == 1 && fwrite(&nrep, 4, 1, f) == 1 && fwrite(&n, 4, 1, f) == 1;
+    for (const rd_system::Rep& A : h->reps) {
+        const uint32_t cnt = (uint32_t)A.n;
+        ok = ok && fwrite(A.lo, 8, 3, f) == 3 && fwrite(A.hi, 8, 3, f) == 3;
+        ok = ok && fwrite(&A.stream, 4, 1, f) == 1 && fwrite(&cnt, 4, 1, f) == 1;
+    }
+    ok = ok && fwrite(h->hxyz.data(), 4, 3 * (size_t)n, f) == 3 * (size_t)n;
+    ok = ok && fwrite(h->htype.data(), 4, (size_t)n, f) == (size_t)n;
+    ok = ok && fwrite(&nb, 4, 1, f) == 1;
+    ok = ok && fwrite(h->bond_pairs.data(), 4, 2 * (size_t)nb, f) == 2 * (size_t)nb;
+    ok = ok && fwrite(h->bond_pot.data(), 4, (size_t)nb, f) == (size_t)nb;
+    if (fclose(f) != 0) ok = false;
+    return ok ? RD_OK : set_err(h, RD_EINVAL, std::string("write failed: ") + path);
+}
+
+extern "C" int rd_checkpoint_read(rd_system* h, const char* path)
+{
+    if (!h || !path) return RD_EINVAL;
+    FILE* f = fopen(path, "rb");
+    if (!f) return set_err(h, RD_EINVAL, std::string("cannot read ") + path);
+    char magic[8];
+    uint32_t version = 0, nt = 0, nrep = 0, n = 0, nb = 0;
+    uint64_t seed = 0, step = 0;
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, RD_CKPT_MAGIC, 8) == 0;
+    ok = ok && fread(&version, 4, 1, f) == 1 && version == 1;
+    ok = ok && fread(&seed, 8, 1, f) == 1 && fread(&step, 8, 1, f) == 1;
+    ok = ok && fread(&nt, 4, 1, f) == 1 && fread(&nrep, 4, 1, f) == 1 && fread(&n, 4, 1, f) == 1;
+    ok = ok && nrep >= 1 && nrep <= (1u << 20) && n <= (uint32_t)RD_MAX_PARTICLES;
+    std::vector<double> bounds;
+    std::vector<uint32_t> streams;
+    std::vector<int> counts;
+    for (uint32_t r = 0; ok && r < nrep; ++r) {
+        double lo[3], hi[3];
+        uint32_t st = 0, cnt = 0;
+        ok = fread(lo, 8, 3, f) == 3 && fread(hi, 8, 3, f) == 3 && fread(&st, 4, 1, f) == 1 && fread(&cnt, 4, 1, f) == 1;
+        for (int d = 0; d < 3; ++d) { bounds.push_back(lo[d]); bounds.push_back(hi[d]); }
+        streams.push_back(st);
+        counts.push_back((int)cnt);
+    }
+    std::vector<float> xyz(ok ? 3 * (size_t)n : 0);
+    std::vector<int> type(ok ? (size_t)n : 0);
+    ok = ok && fread(xyz.data(), 4, xyz.size(), f) == xyz.size() && fread(type.data(), 4, type.size(), f) == type.size();
+    ok = ok && fread(&nb, 4, 1, f) == 1 && nb <= (1u << 28);
+    std::vector<int> bp(ok ? 2 * (size_t)nb : 0), bo(ok ? (size_t)nb : 0);
+    ok = ok && fread(bp.data(), 4, bp.size(), f) == bp.size() && fread(bo.data(), 4, bo.size(), f) == bo.size();
+    fclose(f);
+    if (!ok) return set_err(h, RD_EINVAL, std::string("not a snapshot of this format: ") + path);
+    if ((int)nt != h->nt) return set_err(h, RD_EINVAL, "snapshot has another number of particle types");
+    if (seed != h->prm.seed) return set_err(h, RD_EINVAL, "snapshot has another noise seed");
+    int rc;
+    if (nrep == 1 && streams[0] == h->prm.stream) {
+        bool same_box = true;
+        for (int d = 0; d < 3; ++d)
+            same_box = same_box && bounds[2 * d] == h->prm.lattice_bounds[2 * d] &&
+                       bounds[2 * d + 1] == h->prm.lattice_bounds[2 * d + 1];
+        rc = same_box ? rd_set_particles(h, (int)n, xyz.data(), type.data())
+                      : rd_set_replicas(h, 1, bounds.data(), streams.data(), counts.data(), xyz.data(), type.data());
+    } else {
+        rc = rd_set_replicas(h, (int)nrep, bounds.data(), streams.data(), counts.data(), xyz.data(), type.data());
+    }
+    if (rc) return rc;
+    if (nb > 0) {
+        if (h->bpots.empty()) return set_err(h, RD_ESTATE, "snapshot has bonds: set the bond potentials first");
+        rc = rd_set_bonds(h, (int)nb, bp.data(), bo.data(), (int)h->bpots.size(), h->bpots.data(),
+                          h->bpot_r0.data());
+    } else {
+        h->bond_pairs.clear();
+        h->bond_pot.clear();
+        h->dirty = true;
+    }
+    if (rc) return rc;
+    if (!h->initialized) {
+        h->step = step;
+        return RD_OK;
+    }
+    rc = refresh(h);
+    if (rc) return rc;
+    return rd_set_step(h, step);
 }

@@ -255,6 +255,19 @@ class Simulation:
     def reaction_candidates(self) -> np.ndarray:
         return self._pairs(self.lib.rd_reaction_candidates)
 
+    def checkpoint(self, path: str):
+        """Binary snapshot: particles, replica layout, bonds, seed, step counter."""
+        self._check(self.lib.rd_checkpoint_write(self.h, path.encode()))
+
+    def restore(self, path: str):
+        """Load a snapshot written by a handle with the same model; the run
+        continues bit for bit."""
+        self._check(self.lib.rd_checkpoint_read(self.h, path.encode()))
+
+    def set_step(self, step: int):
+        """Set the step (noise) counter."""
+        self._check(self.lib.rd_set_step(self.h, int(step)))
+
     def simulate_async(self, nsteps: int):
         """Enqueue steps without waiting (errors surface at frame_wait)."""
         self._check(self.lib.rd_simulate_async(self.h, int(nsteps)))
