@@ -243,6 +243,13 @@ int rd_checkpoint_write(rd_system* h, const char* path);
 int rd_checkpoint_read(rd_system* h, const char* path);
 int rd_set_step(rd_system* h, uint64_t step);
 
+/* Debug invariant check of the cell-sorted state (no counterpart in the
+ * reference, whose only integrity checks are the host list walkers of
+ * CudaSimulation.cu:628-726): every particle lies in the slot range of its
+ * cell, every particle index occurs once, cell starts rise to the particle
+ * count.  *violations = number of failed checks (0: consistent). */
+int rd_check_invariants(rd_system* h, long* violations);
+
 /* ---- composite external potential and its learned surrogate ----
  * No counterpart in the reference.  The composite is a geometry of order-one
  * pieces: attractive disks and cylinders (the forms of orderOne,

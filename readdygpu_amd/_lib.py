@@ -128,6 +128,7 @@ _SIGS = {
     "rd_checkpoint_write": (C.c_int, [P, C.c_char_p]),
     "rd_checkpoint_read": (C.c_int, [P, C.c_char_p]),
     "rd_set_step": (C.c_int, [P, C.c_uint64]),
+    "rd_check_invariants": (C.c_int, [P, C.POINTER(C.c_long)]),
     "rd_frame_copy": (C.c_int, [P]),
     "rd_frame_wait": (C.c_int, [P, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]),
     "rd_frame_rewind": (C.c_int, [P]),

@@ -1844,3 +1844,66 @@ extern "C" int rd_checkpoint_read(rd_system* h, const char* path)
     if (rc) return rc;
     return rd_set_step(h, step);
 }
+
+// ---- invariant check of the sorted state (debug aid; no counterpart in the
+// reference, whose only integrity checks are the host list walkers of
+// CudaSimulation.cu:628-726) ----
+// Each slot lies inside the slot range of the cell its position maps to; each
+// particle index occurs once; the cell starts rise and end at the particle
+// count.  Violations are counted.
+__global__ __launch_bounds__(RD_BLOCK) void k_check_slots(const float4* __restrict__ ps,
+                                                          const uint32_t* __restrict__ start,
+                                                          const RdRep* __restrict__ reps,
+                                                          const RdWave* __restrict__ waves, int nwaves,
+                                                          uint32_t* __restrict__ seen, uint32_t* __restrict__ bad)
+{
+    const RdWave it = wave_item(waves, nwaves, wave_id());
+    const int s = it.s0 + (threadIdx.x & 63);
+    if (s >= it.s1) return;
+    const RdRep R = reps[it.rep];
+    const float4 p = ps[s];
+    const uint32_t k = cell_key(R, p.x, p.y, p.z);
+    if (!((uint32_t)s >= start[k] && (uint32_t)s < start[k + 1])) atomicAdd(bad, 1u);
+    const uint32_t li = w_index(p.w);
+    if (li >= (uint32_t)R.n) { atomicAdd(bad, 1u); return; }
+    atomicAdd(&seen[R.off + li], 1u);
+}
+
+__global__ __launch_bounds__(RD_BLOCK) void k_check_counts(const uint32_t* __restrict__ seen, int n,
+                                                           const uint32_t* __restrict__ start, int ncell,
+                                                           uint32_t* __restrict__ bad)
+{
+    const long i = (long)blockIdx.x * RD_BLOCK + threadIdx.x;
+    if (i < n && seen[i] != 1u) atomicAdd(bad, 1u);
+    if (i < ncell && start[i] > start[i + 1]) atomicAdd(bad, 1u);
+    if (i == 0 && start[ncell] != (uint32_t)n) atomicAdd(bad, 1u);
+}
+
+extern "C" int rd_check_invariants(rd_system* h, long* violations)
+{
+    if (!h || !violations) return RD_EINVAL;
+    int rc = refresh(h);
+    if (rc) return rc;
+    uint32_t *d_seen = nullptr, *d_bad = nullptr;
+    hipError_t e = hipMalloc(&d_seen, sizeof(uint32_t) * ((size_t)h->n + 1));
+    if (e == hipSuccess) e = hipMalloc(&d_bad, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(d_seen, 0, sizeof(uint32_t) * ((size_t)h->n + 1), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, sizeof(uint32_t), h->stream);
+    uint32_t bad = 0;
+    if (e == hipSuccess) {
+        if (h->n > 0)
+            hipLaunchKernelGGL(k_check_slots, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
+                               h->d_reps, h->d_waves, h->nwaves, d_seen, d_bad);
+        const long m = std::max<long>(h->n, h->ncell);
+        hipLaunchKernelGGL(k_check_counts, dim3(std::max(1, nblk(m, RD_BLOCK))), dim3(RD_BLOCK), 0, h->stream, d_seen,
+                           h->n, h->d_start, h->ncell, d_bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (d_seen) (void)hipFree(d_seen);
+    if (d_bad) (void)hipFree(d_bad);
+    if (e != hipSuccess) return set_err(h, RD_EHIP, std::string("invariant check: ") + hipGetErrorString(e));
+    *violations = (long)bad;
+    return RD_OK;
+}

@@ -264,6 +264,12 @@ class Simulation:
         continues bit for bit."""
         self._check(self.lib.rd_checkpoint_read(self.h, path.encode()))
 
+    def check_invariants(self) -> int:
+        """Number of violated invariants of the cell-sorted state (0: consistent)."""
+        v = C.c_long()
+        self._check(self.lib.rd_check_invariants(self.h, C.byref(v)))
+        return v.value
+
     def set_step(self, step: int):
         """Set the step (noise) counter."""
         self._check(self.lib.rd_set_step(self.h, int(step)))
