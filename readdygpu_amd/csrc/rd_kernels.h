@@ -126,7 +126,6 @@ struct RdTab {
     float r0[RD_MAXT * RD_MAXT];
     uint32_t mask[RD_MAXT * RD_MAXT];
     float a[RD_MAXT], b[RD_MAXT], prad[RD_MAXT];
-    float rcmax2[RD_MAXT];          // largest squared range of a type with any partner
     RdPot2Dev pot2[RD_MAXP2];
 };
 
@@ -143,10 +142,6 @@ __device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
         T.a[k] = P->a[k];
         T.b[k] = P->b[k];
         T.prad[k] = P->prad[k];
-        float m = 0.0f;
-        if (k < P->nt)
-            for (int j = 0; j < P->nt; ++j) m = fmaxf(m, P->rc2[k * P->nt + j]);
-        T.rcmax2[k] = m;
     }
     const int nw = (int)(sizeof(RdPot2Dev) / 4) * RD_MAXP2;
     for (int k = threadIdx.x; k < nw; k += blockDim.x)
@@ -303,10 +298,10 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
     if (P->n_pot2 > 0) {
         if (P->S == 1 && Lx >= 3 && Ly >= 3 && Lz >= 3) {
             const bool in = interior_cell(R, cx, cy, cz);
-            const float rmax2 = T.rcmax2[ti];
             const int lane = threadIdx.x & 63;
             // Phase 1: every stencil candidate gets the cheap distance test
-            // against the home type's largest range; hits are listed in LDS.
+            // against the range of its type pair (one LDS table read); hits
+            // are listed in LDS.
             // Phase 2 evaluates the listed pairs only, so the interaction
             // code runs (max hits over the wave) times, not once per
             // candidate iteration in which some lane has a hit.
@@ -319,7 +314,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                     dz = min_image(dz, R.Lb[2], R.half[2]);
                 }
                 const float d2 = dist2(dx, dy, dz);
-                if (d2 < rmax2 && d2 > 0.0f) {
+                if (d2 < T.rc2[rowbase + w_type(q.w)] && d2 > 0.0f) {
                     if (nhit < RD_HITS) il->j[nhit][lane] = j;
                     else pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), ax, ay, az, e);
                     ++nhit;
