@@ -71,10 +71,14 @@ def cpu_baseline(spec, xyz, types, budget_s=15.0):
     }
 
 
-def bench_handle(sim, steps, warmup):
+def bench_handle(sim, steps, warmup, tune_steps=0):
     """(wall ms of the graph pass, force ms, sort ms, external-pass ms of the
-    event pass) for `steps` steps after `warmup` steps."""
+    event pass) for `steps` steps after `warmup` steps; with tune_steps the
+    autotuner runs on the warmed-up state (it restores the state, so the
+    trajectory does not change)."""
     sim.simulate(warmup)
+    if tune_steps:
+        sim.autotune(trial_steps=tune_steps)
     out = (C.c_double * 5)()
     rc = sim.lib.rd_bench_steps(sim.h, steps, out)
     if rc != 0:
@@ -90,8 +94,9 @@ def bench_batched(steps=200, warmup=10):
     spec, boxes, counts, xyz, types = workloads.batched_replicas()
     n = int(counts.sum())
     sim = Simulation(spec, xyz, types, replicas=(boxes, counts))
+    sim.simulate(warmup)
     ms_tune = sim.autotune(trial_steps=5)
-    wall, force, sort, _ = bench_handle(sim, steps, warmup)
+    wall, force, sort, _ = bench_handle(sim, steps, 0)
     info = sim.info()
     sim.close()
     batched = n * steps / (wall / 1e3)
@@ -154,10 +159,9 @@ def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000):
         f"rel force RMSE {rmse:.4f}")
     w = sim.surrogate_weights()
     lo, hi, gsc = sim.surrogate_norm()
-    sim.autotune(trial_steps=3)
     # exact external pass (this handle), then a fresh handle with the same
     # start, seed and step counter running the surrogate: identical noise
-    wall_e, force_e, sort_e, ext_e = bench_handle(sim, steps, warmup)
+    wall_e, force_e, sort_e, ext_e = bench_handle(sim, steps, warmup, tune_steps=3)
     sim.call_rdf_calculation()
     g_exact = sim.normalize_rdf()
     sim.close()
@@ -166,8 +170,7 @@ def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000):
     sim.set_surrogate_weights(w)
     sim.set_surrogate_norm(lo, hi, gsc)
     sim.set_external_mode(True)
-    sim.autotune(trial_steps=3)
-    wall_s, force_s, sort_s, ext_s = bench_handle(sim, steps, warmup)
+    wall_s, force_s, sort_s, ext_s = bench_handle(sim, steps, warmup, tune_steps=3)
     sim.call_rdf_calculation()
     g_sur = sim.normalize_rdf()
     sim.close()
@@ -446,8 +449,7 @@ def bench_cytosol(steps=100, warmup=10, rdf_steps=200, frames=10, plain_steps=20
     for mode in ("exact", "surrogate"):
         if mode == "surrogate":
             sim = handle(True, w, norm)
-        sim.autotune(trial_steps=2)
-        wall, force, sort, ext = bench_handle(sim, steps, warmup)
+        wall, force, sort, ext = bench_handle(sim, steps, warmup, tune_steps=2)
         out[mode] = {"value": n * steps / (wall / 1e3), "ms_per_step": wall / steps,
                      "ext_avg_us": ext / steps * 1e3, "force_avg_us": force / steps * 1e3,
                      "sort_avg_us": sort / steps * 1e3}
