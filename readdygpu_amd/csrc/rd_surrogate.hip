@@ -538,10 +538,21 @@ __global__ __launch_bounds__(256) void k_mlp_adam(const float* __restrict__ part
                                                   float* __restrict__ grad_out, int apply,
                                                   float* __restrict__ loss_hist)
 {
-    int p = blockIdx.x * 256 + threadIdx.x;
-    if (p > RD_MLP_NPARAM) return;
+    // 32 parameters per workgroup, the per-wave partial rows split over 8
+    // slices, the slices added in a fixed order: deterministic, and enough
+    // workgroups to keep the loads in flight
+    __shared__ float red[8][32];
+    const int pl = threadIdx.x & 31, slice = threadIdx.x >> 5;
+    const int p = blockIdx.x * 32 + pl;
+    float part = 0.0f;
+    if (p <= RD_MLP_NPARAM)
+        for (int w = slice; w < nw; w += 8) part += partial[(size_t)w * RD_MLP_ROW + p];
+    red[slice][pl] = part;
+    __syncthreads();
+    if (slice != 0 || p > RD_MLP_NPARAM) return;
     float gsum = 0.0f;
-    for (int w = 0; w < nw; ++w) gsum += partial[(size_t)w * RD_MLP_ROW + p];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gsum += red[k][pl];
     if (p == RD_MLP_NPARAM) {        // loss column
         ctl->loss = gsum;
         if (loss_hist) loss_hist[ctl->step] = gsum;
@@ -1038,7 +1049,7 @@ static void launch_train_step(rd_system* h, const float4* xn, const float4* tn, 
     const int blocks = (batch + 255) / 256;
     hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), train_lds(), h->stream, xn, tn, m, batch, sequential,
                        seed, (const RdMlpPack*)h->d_pack, (RdTrainCtl*)h->d_ctl, h->d_partial);
-    hipLaunchKernelGGL(k_mlp_adam, dim3(nb256(RD_MLP_NPARAM + 1)), dim3(256), 0, h->stream, h->d_partial, blocks * 4,
+    hipLaunchKernelGGL(k_mlp_adam, dim3((RD_MLP_NPARAM + 1 + 31) / 32), dim3(256), 0, h->stream, h->d_partial, blocks * 4,
                        h->d_master, h->d_mom, h->d_vel, (RdMlpPack*)h->d_pack, (RdTrainCtl*)h->d_ctl, tp.lr, tp.lr_min,
                        (long long)tp.steps, tp.beta1, tp.beta2, tp.eps, grad_out, apply, loss_hist);
     if (apply) hipLaunchKernelGGL(k_mlp_step_inc, dim3(1), dim3(1), 0, h->stream, (RdTrainCtl*)h->d_ctl);
