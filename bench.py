@@ -98,6 +98,7 @@ def bench_batched(steps=200, warmup=10):
     ms_tune = sim.autotune(trial_steps=5)
     wall, force, sort, _ = bench_handle(sim, steps, 0)
     info = sim.info()
+    lc = sim.launch_config()
     sim.close()
     batched = n * steps / (wall / 1e3)
     # the plain form: each replica on its own handle, all steps of one before the next
@@ -123,7 +124,8 @@ def bench_batched(steps=200, warmup=10):
         "value": batched, "unit": "particle-steps/s", "ms_per_step": wall / steps,
         "force_avg_us": force / steps * 1e3, "sort_avg_us": sort / steps * 1e3,
         "plain_per_replica_value": separate, "speedup_vs_plain": batched / separate,
-        "cell_size_nm": info.cell_size, "autotune_us_per_step": ms_tune * 1e3,
+        "cell_size_nm": info.cell_size, "workgroup": lc["used_block"], "particles_per_lane": lc["used_ppt"],
+        "autotune_us_per_step": ms_tune * 1e3,
     }
 
 
@@ -515,6 +517,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
     ap.add_argument("--kernel", type=int, default=-1, help="force kernel: 1 gather")
     ap.add_argument("--block", type=int, default=0, help="force-kernel workgroup size (64-512; 0: autotuned)")
+    ap.add_argument("--ppt", type=int, default=0, help="force-kernel particles per lane (1, 2, 4; 0: autotuned)")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned workloads")
     ap.add_argument("--only", choices=["batched_replicas", "learned_slab", "learned_membrane", "crowded_cytosol",
                                        "frame_handoff"],
@@ -549,14 +552,14 @@ def main():
     n = len(types)
     log(f"setting up plain_1m n={n}")
     sim = Simulation(spec, xyz, types)
-    if args.kernel >= 0 or args.block:
-        sim.set_launch_config(kernel=max(args.kernel, 0), block=args.block)
+    if args.kernel >= 0 or args.block or args.ppt:
+        sim.set_launch_config(kernel=max(args.kernel, 0), block=args.block, ppt=args.ppt)
     # warm up first: the confinement pulls the particles outside the sphere
     # onto its surface within the first steps, so the tuned cell size has to
     # be chosen on the shell that the timed steps see (the autotuner restores
     # the state it starts from)
     sim.simulate(args.warmup)
-    if not args.no_tune and args.kernel < 0 and args.cell == 0.0 and not args.block:
+    if not args.no_tune and args.kernel < 0 and args.cell == 0.0 and not args.block and not args.ppt:
         ms = sim.autotune(trial_steps=20)
         log(f"autotuned: {ms * 1e3:.1f} us/step")
     info = sim.info()
@@ -622,6 +625,8 @@ def main():
             "cell_size_nm": info.cell_size,
             "lattice": list(info.lattice),
             "force_kernel": {1: "gather"}.get(lc["used_kernel"], "?"),
+            "workgroup": lc["used_block"],
+            "particles_per_lane": lc["used_ppt"],
         },
         "roofline": roofline,
         "passes": passes,

@@ -350,15 +350,30 @@ typedef struct {
     int used_kernel;       /* out: kernel in use */
     int block;             /* threads per workgroup of the force kernel: 64, 128, 256 or 512; 0 = 256 */
     int used_block;        /* out: workgroup size in use */
+    int ppt;               /* particles per lane of the force kernel: 1, 2 or 4 (a wave takes that many
+                              consecutive 64-slot work items in turn); 0 = 1 */
+    int used_ppt;          /* out: particles per lane in use */
 } rd_launch_config;
 int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg);
 int rd_get_launch_config(rd_system* h, rd_launch_config* cfg);
 
 /* Autotuner: times `trial_steps` steps for each candidate cell size (factors
- * of the interaction range) and force-kernel workgroup size, keeps the
- * fastest, and restores the particle state and step counter, so tuning does
- * not change the run. */
+ * of the interaction range) and force-kernel workgroup size, then particles
+ * per lane at the winning pair; keeps the fastest, and restores the particle
+ * state and step counter, so tuning does not change the run.  Values fixed
+ * by rd_set_launch_config (block, ppt) or the cell size are kept. */
 int rd_autotune(rd_system* h, int trial_steps, double* best_ms_per_step);
+
+/* Lattice cell edge of a running system (<= 0: 1.01 x the largest interaction
+ * range); the particles keep their positions and the step counter is kept.
+ * RD_ERANGE (previous lattice kept) when the lattice would be too large.  No
+ * counterpart in the reference (its cell edge is fixed at creation,
+ * CudaSimulation.cu:522-560); used to apply a cached autotuner choice. */
+int rd_set_cell_size(rd_system* h, double cell);
+
+/* "<name> <arch> <N>CU" of a HIP device, the key of cached autotuner choices.
+ * No counterpart in the reference. */
+int rd_device_name(int device, char* buf, int cap);
 
 #ifdef __cplusplus
 }

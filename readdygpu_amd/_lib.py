@@ -58,6 +58,8 @@ class rd_launch_config(C.Structure):
         ("used_kernel", C.c_int),
         ("block", C.c_int),
         ("used_block", C.c_int),
+        ("ppt", C.c_int),
+        ("used_ppt", C.c_int),
     ]
 
 
@@ -153,6 +155,8 @@ _SIGS = {
     "rd_set_launch_config": (C.c_int, [P, C.POINTER(rd_launch_config)]),
     "rd_get_launch_config": (C.c_int, [P, C.POINTER(rd_launch_config)]),
     "rd_autotune": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
+    "rd_set_cell_size": (C.c_int, [P, C.c_double]),
+    "rd_device_name": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
     "rd_model_read_xml": (C.c_int, [C.c_char_p] * 6 + [C.POINTER(P), C.c_char_p, C.c_int]),
     "rd_model_free": (None, [P]),
     "rd_model_get_info": (C.c_int, [P, C.POINTER(rd_model_info)]),

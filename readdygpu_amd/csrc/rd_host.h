@@ -132,6 +132,7 @@ struct rd_system {
     rd_launch_config lc{};        // requested
     int kern = RD_KERNEL_GATHER;  // kernel in use (RD_KERNEL_*)
     int block = 256;              // force-kernel workgroup size in use
+    int ppt = 1;                  // force-kernel particles per lane in use
 
     // composite external potential (nearest piece) and its MLP surrogate
     std::vector<rd_pot1> comp;

@@ -42,9 +42,10 @@ __device__ __forceinline__ int wave_id() { return xcd_block(blockIdx.x, gridDim.
 // One BD step for the particle in slot s with the per-particle global gather
 // (CudaSimulation.cu:2314-2465 fused): gradient, Euler-Maruyama
 // (CudaSimulation.cu:1852), periodic wrap (:1857-1858), next cell key.
-// BLOCK threads per workgroup (the autotuner's choice); the type-pair table is
-// staged in LDS once per workgroup.
-template <int BLOCK>
+// BLOCK threads per workgroup and PPT particles per lane (the autotuner's
+// choices; a wave takes PPT consecutive 64-slot work items one after another);
+// the type-pair table is staged in LDS once per workgroup.
+template <int BLOCK, int PPT>
 __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict__ ps, float4* __restrict__ pn,
                                                            uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                                                            uint32_t* __restrict__ count,
@@ -59,14 +60,27 @@ __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict
     __shared__ RdTab T;
     __shared__ RdHitList il[BLOCK / 64];
     load_tab(T, P);
-    const int w = xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64) + (threadIdx.x >> 6);
-    const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w));
-    const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
-    if (s >= (uint32_t)it.s1) return;
-    const RdRep R = reps[it.rep];
-    float4 p = ps[s];
-    float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
-    em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count);
+    const int w = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64) + (threadIdx.x >> 6)) * PPT;
+    if constexpr (PPT == 1) {
+        const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w));
+        const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
+        if (s >= (uint32_t)it.s1) return;
+        const RdRep R = reps[it.rep];
+        float4 p = ps[s];
+        float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
+        em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count);
+    } else {
+        for (int k = 0; k < PPT; ++k) {
+            const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w + k));
+            const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
+            if (s < (uint32_t)it.s1) {
+                const RdRep R = reps[it.rep];
+                float4 p = ps[s];
+                float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
+                em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count);
+            }
+        }
+    }
 }
 
 // gradient + energy (FP64 accumulation) at the current positions
@@ -941,22 +955,32 @@ static void launch_sort(rd_system* h, int advance)
 
 static int wave_blocks(const rd_system* h) { return std::max(1, nblk(h->nwaves, RD_BLOCK / 64)); }
 
-template <int B>
+template <int B, int PPT>
 static void launch_force(rd_system* h)
 {
-    hipLaunchKernelGGL(k_force_update<B>, dim3(std::max(1, nblk(h->nwaves, B / 64))), dim3(B), 0, h->stream, h->d_ps,
-                       h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves,
-                       h->d_pot1, h->d_st, h->d_noise, h->n);
+    hipLaunchKernelGGL((k_force_update<B, PPT>), dim3(std::max(1, nblk(h->nwaves, B / 64 * PPT))), dim3(B), 0,
+                       h->stream, h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps,
+                       h->d_waves, h->nwaves, h->d_pot1, h->d_st, h->d_noise, h->n);
+}
+
+template <int B>
+static void launch_force_ppt(rd_system* h)
+{
+    switch (h->ppt) {
+    case 2: launch_force<B, 2>(h); break;
+    case 4: launch_force<B, 4>(h); break;
+    default: launch_force<B, 1>(h); break;
+    }
 }
 
 void rd_host_launch_force(rd_system* h)
 {
     if (h->n <= 0) return;
     switch (h->block) {
-    case 64: launch_force<64>(h); break;
-    case 128: launch_force<128>(h); break;
-    case 512: launch_force<512>(h); break;
-    default: launch_force<256>(h); break;
+    case 64: launch_force_ppt<64>(h); break;
+    case 128: launch_force_ppt<128>(h); break;
+    case 512: launch_force_ppt<512>(h); break;
+    default: launch_force_ppt<256>(h); break;
     }
 }
 
@@ -1033,6 +1057,7 @@ static int upload(rd_system* h)
     h->P.ext_mask = h->comp_mask;
     h->kern = RD_KERNEL_GATHER;
     h->block = (h->lc.block == 64 || h->lc.block == 128 || h->lc.block == 512) ? h->lc.block : 256;
+    h->ppt = (h->lc.ppt == 2 || h->lc.ppt == 4) ? h->lc.ppt : 1;
     HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
     HIPCK(h, hipMemcpyAsync(h->d_reps, h->rep_dev.data(), sizeof(RdRep) * h->rep_dev.size(), hipMemcpyHostToDevice,
                             h->stream));
@@ -1187,6 +1212,8 @@ extern "C" int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg)
     if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_GATHER) return set_err(h, RD_EINVAL, "bad kernel");
     if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256 && cfg->block != 512)
         return set_err(h, RD_EINVAL, "workgroup size must be 64, 128, 256 or 512");
+    if (cfg->ppt != 0 && cfg->ppt != 1 && cfg->ppt != 2 && cfg->ppt != 4)
+        return set_err(h, RD_EINVAL, "particles per lane must be 1, 2 or 4");
     h->lc = *cfg;
     h->dirty = true;
     return RD_OK;
@@ -1198,6 +1225,7 @@ extern "C" int rd_get_launch_config(rd_system* h, rd_launch_config* cfg)
     *cfg = h->lc;
     cfg->used_kernel = h->kern;
     cfg->used_block = h->block;
+    cfg->used_ppt = h->ppt;
     return RD_OK;
 }
 

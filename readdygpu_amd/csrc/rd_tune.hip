@@ -1,9 +1,11 @@
 // rd_tune.hip -- autotuner of the lattice cell size and the force-kernel
-// launch shape.  Every candidate computes the same trajectory bit for bit
+// launch shape (workgroup size, particles per lane).  Every candidate computes the same trajectory bit for bit
 // (fixed-point pair sums), so the choice only affects speed.
 #include "rd_host.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -50,39 +52,52 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     if (user_cell > 0 || base <= 0) cells.push_back(user_cell > 0 ? user_cell : h->cell);
     else
         for (double f : { 1.01, 1.2, 1.45, 1.75, 2.1 }) cells.push_back(f * base);
-    // workgroup sizes of the force kernel (a size the caller fixed is kept)
-    struct Cfg { int kernel, block; };
+    // workgroup sizes of the force kernel (a value the caller fixed is kept),
+    // at one particle per lane; particles per lane are tried afterwards at
+    // the winning cell and workgroup size
+    struct Cfg { int kernel, block, ppt; };
     std::vector<Cfg> cfgs;
+    const int ppt0 = user_lc.ppt ? user_lc.ppt : 1;
     for (int b : { 64, 128, 256, 512 })
-        if (user_lc.block == 0 || user_lc.block == b) cfgs.push_back({ RD_KERNEL_GATHER, b });
+        if (user_lc.block == 0 || user_lc.block == b) cfgs.push_back({ RD_KERNEL_GATHER, b, ppt0 });
     double best = 1e30, best_cell = cells[0];
     Cfg best_cfg = cfgs[0];
-    for (double c : cells) {
-        for (const Cfg& k : cfgs) {
-            h->prm.cell_size = c;
-            rd_launch_config lc{};
-            lc.kernel = k.kernel;
-            lc.block = k.block;
-            rc = rd_set_launch_config(h, &lc);
-            if (rc == RD_OK) rc = rd_host_set_state(h, sx, st);
-            if (rc == RD_OK) rc = rd_host_refresh(h);
-            if (rc == RD_ERANGE) continue;          // too many cells for this size
-            if (rc) return rc;
-            // best of two timings: a single short one is at the mercy of clock
-            // and cache state on candidates that differ by a few percent
-            double ms = 0.0, ms2 = 0.0;
-            rc = time_steps(h, trial_steps, &ms);
-            if (rc == RD_OK) rc = time_steps(h, trial_steps, &ms2);
-            if (rc) return rc;
-            ms = std::min(ms, ms2);
-            if (ms < best) { best = ms; best_cell = c; best_cfg = k; }
-        }
+    auto trial = [&](double c, const Cfg& k) -> int {
+        h->prm.cell_size = c;
+        rd_launch_config lc{};
+        lc.kernel = k.kernel;
+        lc.block = k.block;
+        lc.ppt = k.ppt;
+        int r = rd_set_launch_config(h, &lc);
+        if (r == RD_OK) r = rd_host_set_state(h, sx, st);
+        if (r == RD_OK) r = rd_host_refresh(h);
+        if (r == RD_ERANGE) return RD_OK;       // too many cells for this size
+        if (r) return r;
+        // best of two timings: a single short one is at the mercy of clock
+        // and cache state on candidates that differ by a few percent
+        double ms = 0.0, ms2 = 0.0;
+        r = time_steps(h, trial_steps, &ms);
+        if (r == RD_OK) r = time_steps(h, trial_steps, &ms2);
+        if (r) return r;
+        ms = std::min(ms, ms2);
+        if (ms < best) { best = ms; best_cell = c; best_cfg = k; }
+        return RD_OK;
+    };
+    for (double c : cells)
+        for (const Cfg& k : cfgs)
+            if ((rc = trial(c, k)) != RD_OK) return rc;
+    if (user_lc.ppt == 0) {
+        const Cfg first = best_cfg;
+        const double c = best_cell;
+        for (int q : { 2, 4 })
+            if ((rc = trial(c, Cfg{ first.kernel, first.block, q })) != RD_OK) return rc;
     }
     // keep the winner, restore the state
     h->prm.cell_size = user_cell > 0 ? user_cell : best_cell;
     rd_launch_config lc = user_lc;
     lc.kernel = best_cfg.kernel;
     lc.block = best_cfg.block;
+    lc.ppt = best_cfg.ppt;
     rc = rd_set_launch_config(h, &lc);
     if (rc == RD_OK) rc = rd_host_set_state(h, sx, st);
     if (rc == RD_OK) rc = rd_host_refresh(h);
@@ -90,5 +105,39 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     rc = rd_host_set_step(h, s0);
     if (rc) return rc;
     if (best_ms) *best_ms = best;
+    return RD_OK;
+}
+
+extern "C" int rd_set_cell_size(rd_system* h, double cell)
+{
+    if (!h || !(cell >= 0.0) || !std::isfinite(cell)) return RD_EINVAL;
+    if (!h->initialized) {
+        h->prm.cell_size = cell;
+        h->dirty = true;
+        return RD_OK;
+    }
+    int rc = rd_copy_pos_from_device(h, nullptr, nullptr);
+    if (rc) return rc;
+    const std::vector<float> sx = h->hxyz;
+    const std::vector<int> st = h->htype;
+    const double old = h->prm.cell_size;
+    h->prm.cell_size = cell;
+    rc = rd_host_set_state(h, sx, st);
+    if (rc == RD_OK) rc = rd_host_refresh(h);
+    if (rc == RD_ERANGE) {                       // lattice too large: keep the previous one
+        h->prm.cell_size = old;
+        int r2 = rd_host_set_state(h, sx, st);
+        if (r2 == RD_OK) r2 = rd_host_refresh(h);
+        return r2 ? r2 : RD_ERANGE;
+    }
+    return rc;
+}
+
+extern "C" int rd_device_name(int device, char* buf, int cap)
+{
+    if (!buf || cap < 1) return RD_EINVAL;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return RD_EHIP;
+    snprintf(buf, (size_t)cap, "%s %s %dCU", prop.name, prop.gcnArchName, prop.multiProcessorCount);
     return RD_OK;
 }

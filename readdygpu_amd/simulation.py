@@ -9,6 +9,10 @@ nonzero status, as the reference's methods return 1 and print.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
+import hashlib
+import json
+import os
 from typing import Optional
 
 import numpy as np
@@ -39,6 +43,8 @@ class Simulation:
                  rdf: Optional[tuple] = None, initialize: bool = True, replicas: Optional[tuple] = None):
         self.lib = _lib.load()
         self.spec = spec
+        self.device = device
+        self._tune_tags = {}          # what else shapes a step's cost, for the tuning cache key
         prm = _lib.rd_params()
         for d in range(3):
             prm.lattice_bounds[2 * d] = spec.box_lo[d]
@@ -146,6 +152,7 @@ class Simulation:
             if len(r) != len(pots):
                 raise ValueError("r0 needs one value per bond potential")
         self._check(self.lib.rd_set_bonds(self.h, nb, _ptr(b), _ptr(pi), len(pots), arr, _ptr(r)))
+        self._tune_tags["bonds"] = nb
 
     def set_replicas(self, bounds: np.ndarray, counts: np.ndarray, xyz: np.ndarray, types: np.ndarray,
                      streams: Optional[np.ndarray] = None):
@@ -162,6 +169,7 @@ class Simulation:
         if st is not None and len(st) != len(c):
             raise ValueError("streams and counts differ in length")
         self._check(self.lib.rd_set_replicas(self.h, len(c), _ptr(b), _ptr(st), _ptr(c), _ptr(xyz), _ptr(types)))
+        self._tune_tags["replicas"] = hashlib.sha1(b.tobytes() + c.tobytes()).hexdigest()
 
     def initialize(self):
         """CudaSimulation::initialize (CudaSimulation.cu:138-455)."""
@@ -313,12 +321,14 @@ class Simulation:
                 raise ValueError("creation_replica needs one entry per reaction")
             self._check(self.lib.rd_apply_reactions_replicas(self.h, _ptr(cr), _ptr(rr)))
 
-    def set_launch_config(self, kernel: int = 0, block: int = 0):
-        """Force kernel (0 automatic, 1 per-particle gather) and its workgroup
-        size (64, 128, 256, 512; 0 = 256)."""
+    def set_launch_config(self, kernel: int = 0, block: int = 0, ppt: int = 0):
+        """Force kernel (0 automatic, 1 per-particle gather), its workgroup
+        size (64, 128, 256, 512; 0 = 256) and particles per lane (1, 2, 4;
+        0 = 1)."""
         c = _lib.rd_launch_config()
         c.kernel = int(kernel)
         c.block = int(block)
+        c.ppt = int(ppt)
         self._check(self.lib.rd_set_launch_config(self.h, C.byref(c)))
 
     def launch_config(self) -> dict:
@@ -326,11 +336,75 @@ class Simulation:
         self._check(self.lib.rd_get_launch_config(self.h, C.byref(c)))
         return {k: getattr(c, k) for k, _ in c._fields_}
 
-    def autotune(self, trial_steps: int = 20) -> float:
-        """Pick the cell size and kernel shape by timing; state is restored."""
+    def autotune(self, trial_steps: int = 20, cache: Optional[str] = None) -> float:
+        """Pick the cell size, workgroup size and particles per lane by timing;
+        state is restored.  ``cache`` (default: $RDGPU_TUNE_CACHE) names a JSON
+        file of earlier choices keyed by the device and the workload (spec,
+        particle count, replica layout, bonds, composite): a hit applies the
+        stored choice without timing and returns its recorded ms/step, a miss
+        tunes and adds the choice to the file."""
+        path = cache if cache is not None else os.environ.get("RDGPU_TUNE_CACHE")
+        key = self.tune_key() if path else None
+        db = {}
+        if path and os.path.exists(path):
+            try:
+                with open(path) as f:
+                    db = json.load(f)
+            except (OSError, ValueError):
+                db = {}
+            hit = db.get(key) if isinstance(db, dict) else None
+            if hit:
+                self.set_launch_config(kernel=1, block=int(hit["block"]), ppt=int(hit["ppt"]))
+                self._check(self.lib.rd_set_cell_size(self.h, float(hit["cell_size"])))
+                return float(hit["ms_per_step"])
         ms = C.c_double()
         self._check(self.lib.rd_autotune(self.h, int(trial_steps), C.byref(ms)))
+        if path:
+            lc = self.launch_config()
+            db = db if isinstance(db, dict) else {}
+            db[key] = {"cell_size": self.info().cell_size, "block": lc["used_block"], "ppt": lc["used_ppt"],
+                       "ms_per_step": ms.value}
+            tmp = path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(db, f, indent=1, sort_keys=True)
+            os.replace(tmp, path)
         return ms.value
+
+    def set_cell_size(self, cell: float):
+        """Lattice cell edge of the running system (<= 0: automatic); positions
+        and the step counter are kept."""
+        self._check(self.lib.rd_set_cell_size(self.h, float(cell)))
+
+    def tune_key(self) -> str:
+        """Key of this workload in the tuning cache: device name and a digest of
+        the spec, the particle count and the replica/bond/composite layout."""
+        buf = C.create_string_buffer(256)
+        self._check(self.lib.rd_device_name(int(self.device), buf, 256))
+        h = hashlib.sha1()
+
+        def feed(v):
+            if dataclasses.is_dataclass(v) and not isinstance(v, type):
+                for f in dataclasses.fields(v):
+                    h.update(f.name.encode())
+                    feed(getattr(v, f.name))
+            elif isinstance(v, np.ndarray):
+                h.update(str(v.dtype).encode() + str(v.shape).encode())
+                h.update(np.ascontiguousarray(v).tobytes())
+            elif isinstance(v, (list, tuple)):
+                h.update(b"[")
+                for x in v:
+                    feed(x)
+                h.update(b"]")
+            elif isinstance(v, dict):
+                for k in sorted(v):
+                    h.update(str(k).encode())
+                    feed(v[k])
+            else:
+                h.update(repr(v).encode())
+        feed(self.spec)
+        feed(int(self.info().n))
+        feed(self._tune_tags)
+        return f"{buf.value.decode()}|{h.hexdigest()}"
 
     # -- composite external potential and surrogate (no reference counterpart) --
     def set_composite(self, pieces, types=None):
@@ -348,9 +422,11 @@ class Simulation:
             q.type_mask = 0xFFFFFFFF
         mask = 0xFFFFFFFF if types is None else sum(1 << int(t) for t in types)
         self._check(self.lib.rd_set_composite(self.h, len(pieces), C.cast(arr, C.c_void_p), mask))
+        self._tune_tags["composite"] = [len(pieces), mask]
 
     def set_external_mode(self, surrogate: bool):
         self._check(self.lib.rd_set_external_mode(self.h, 1 if surrogate else 0))
+        self._tune_tags["surrogate"] = bool(surrogate)
 
     def composite_eval(self, xyz):
         """Exact composite on the GPU at positions [m,3] -> (grad [m,3], energy [m])."""
