@@ -16,6 +16,8 @@
 #define RD_BLOCK 256
 #define RD_SCAN_ITEMS 8
 #define RD_SCAN_TILE (RD_BLOCK * RD_SCAN_ITEMS)
+#define RD_SCAN_TILE_LOG2 11
+static_assert(RD_SCAN_TILE == (1 << RD_SCAN_TILE_LOG2), "scan tile is a power of two");
 
 struct RdStepState {
     unsigned long long step;        // global step counter (noise counter)
@@ -36,6 +38,25 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
 __device__ __forceinline__ int lanes_below(uint64_t b)
 {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
+// Histogram entry of cell k and of its scan tile (the per-tile sums the scan
+// of the cell counts starts from).  Lanes of a wave are in slot order, so
+// their cells fall in one or two tiles: one atomic per distinct tile, issued
+// by the lowest lane of the tile's group.
+__device__ __forceinline__ uint32_t cell_histogram(uint32_t* __restrict__ count, uint32_t* __restrict__ tsum,
+                                                   uint32_t k)
+{
+    const uint32_t t = k >> RD_SCAN_TILE_LOG2;
+    uint64_t todo = __ballot(1);
+    while (todo) {
+        const int lead = __builtin_ctzll(todo);
+        const uint32_t t0 = (uint32_t)__shfl((int)t, lead, 64);
+        const uint64_t m = __ballot(t == t0) & todo;
+        if ((int)(threadIdx.x & 63) == lead) atomicAdd(&tsum[t0], (uint32_t)__popcll(m));
+        todo &= ~m;
+    }
+    return atomicAdd(&count[k], 1u);
 }
 
 // orders a wave's LDS writes before its following LDS reads by other lanes
@@ -389,7 +410,7 @@ __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, con
                                                 RdStepState* __restrict__ st, const float* __restrict__ noise,
                                                 int ntot, uint32_t s, float4 p, float3 g, float4* __restrict__ pn,
                                                 uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
-                                                uint32_t* __restrict__ count)
+                                                uint32_t* __restrict__ count, uint32_t* __restrict__ tsum)
 {
     const int t = w_type(p.w);
     const uint32_t idx = w_index(p.w);
@@ -419,7 +440,7 @@ __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, con
     if (P->gid_n) P->gid_n[s] = (uint32_t)R.off + idx;
     uint32_t k = cell_key(R, x, y, zc);
     key[s] = k;
-    rank[s] = atomicAdd(&count[k], 1u);
+    rank[s] = cell_histogram(count, tsum, k);
 }
 
 // the wave's work item: replica and slot range (waves never span replicas)

@@ -23,7 +23,8 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pack(const float* __restrict__ xyz
                                                    const int* __restrict__ rep_of, int n,
                                                    const RdRep* __restrict__ reps, float4* __restrict__ pn,
                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
-                                                   uint32_t* __restrict__ count, uint32_t* __restrict__ gid_n)
+                                                   uint32_t* __restrict__ count, uint32_t* __restrict__ tsum,
+                                                   uint32_t* __restrict__ gid_n)
 {
     int i = blockIdx.x * RD_BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -33,7 +34,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pack(const float* __restrict__ xyz
     if (gid_n) gid_n[i] = (uint32_t)i;
     uint32_t k = cell_key(R, x, y, z);
     key[i] = k;
-    rank[i] = atomicAdd(&count[k], 1u);
+    rank[i] = cell_histogram(count, tsum, k);
 }
 
 // wave index and lane of this thread; blocks of RD_BLOCK = 4 waves
@@ -48,7 +49,7 @@ __device__ __forceinline__ int wave_id() { return xcd_block(blockIdx.x, gridDim.
 template <int BLOCK, int PPT>
 __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict__ ps, float4* __restrict__ pn,
                                                            uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
-                                                           uint32_t* __restrict__ count,
+                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ tsum,
                                                            const uint32_t* __restrict__ start,
                                                            const RdDev* __restrict__ P,
                                                            const RdRep* __restrict__ reps,
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict
         const RdRep R = reps[it.rep];
         float4 p = ps[s];
         float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
-        em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count);
+        em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count, tsum);
     } else {
         for (int k = 0; k < PPT; ++k) {
             const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w + k));
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict
                 const RdRep R = reps[it.rep];
                 float4 p = ps[s];
                 float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
-                em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count);
+                em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count, tsum);
             }
         }
     }
@@ -240,9 +241,11 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__
                                                       const uint32_t* __restrict__ rank,
                                                       const uint32_t* __restrict__ start, float4* __restrict__ ps,
                                                       int n, RdStepState* __restrict__ st, int advance,
-                                                      const uint32_t* __restrict__ gid_n, uint32_t* __restrict__ slot_of)
+                                                      const uint32_t* __restrict__ gid_n, uint32_t* __restrict__ slot_of,
+                                                      uint32_t* __restrict__ tsum, int ntile)
 {
     int s = blockIdx.x * RD_BLOCK + threadIdx.x;
+    for (int t = s; t < ntile; t += gridDim.x * RD_BLOCK) tsum[t] = 0u;
     if (advance && s == 0) st->step += 1ull;
     if (s >= n) return;
     uint32_t dst = start[key[s]] + rank[s];
@@ -915,6 +918,7 @@ static int ensure_capacity(rd_system* h)
         h->nscan = nblk((long)nc, RD_SCAN_TILE);
         HIPCK(h, hipMalloc(&h->d_bsum, sizeof(uint32_t) * (h->nscan + 16)));
         HIPCK(h, hipMemsetAsync(h->d_count, 0, sizeof(uint32_t) * (nc + 16), h->stream));
+        HIPCK(h, hipMemsetAsync(h->d_bsum, 0, sizeof(uint32_t) * (h->nscan + 16), h->stream));
         h->ncell_alloc = h->ncell;
     }
     if ((int)h->rep_dev.size() > h->nrep_alloc) {
@@ -941,16 +945,16 @@ static int ensure_capacity(rd_system* h)
     return RD_OK;
 }
 
-// scan of count -> start, then scatter pn -> ps
+// scan of count -> start (the per-tile sums were accumulated with the
+// histogram), then scatter pn -> ps, clearing the tile sums for the next step
 static void launch_sort(rd_system* h, int advance)
 {
     int nc = h->ncell;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum);
     hipLaunchKernelGGL(k_scan_apply, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum,
                        h->d_start);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(h->n, 1), RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_pn,
                        h->d_key, h->d_rank, h->d_start, h->d_ps, h->n, h->d_st, advance,
-                       (const uint32_t*)h->d_gid, h->d_slot);
+                       (const uint32_t*)h->d_gid, h->d_slot, h->d_bsum, h->nscan);
 }
 
 static int wave_blocks(const rd_system* h) { return std::max(1, nblk(h->nwaves, RD_BLOCK / 64)); }
@@ -959,7 +963,8 @@ template <int B, int PPT>
 static void launch_force(rd_system* h)
 {
     hipLaunchKernelGGL((k_force_update<B, PPT>), dim3(std::max(1, nblk(h->nwaves, B / 64 * PPT))), dim3(B), 0,
-                       h->stream, h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_P, h->d_reps,
+                       h->stream, h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_bsum, h->d_start, h->d_P,
+                       h->d_reps,
                        h->d_waves, h->nwaves, h->d_pot1, h->d_st, h->d_noise, h->n);
 }
 
@@ -1090,7 +1095,7 @@ static int upload(rd_system* h)
                                     h->stream));
         }
         hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io, h->d_type_io,
-                           h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_gid);
+                           h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_bsum, h->d_gid);
     }
     launch_sort(h, 0);
     HIPCK(h, hipGetLastError());
@@ -1737,7 +1742,7 @@ extern "C" int rd_frame_rewind(rd_system* h)
     if (h->n > 0)
         hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io,
                            h->d_type_io, h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count,
-                           h->d_gid);
+                           h->d_bsum, h->d_gid);
     launch_sort(h, 0);
     HIPCK(h, hipGetLastError());
     HIPCK(h, hipStreamSynchronize(h->stream));
@@ -1899,12 +1904,17 @@ __global__ __launch_bounds__(RD_BLOCK) void k_check_slots(const float4* __restri
 
 __global__ __launch_bounds__(RD_BLOCK) void k_check_counts(const uint32_t* __restrict__ seen, int n,
                                                            const uint32_t* __restrict__ start, int ncell,
+                                                           const uint32_t* __restrict__ count,
+                                                           const uint32_t* __restrict__ tsum, int ntile,
                                                            uint32_t* __restrict__ bad)
 {
     const long i = (long)blockIdx.x * RD_BLOCK + threadIdx.x;
     if (i < n && seen[i] != 1u) atomicAdd(bad, 1u);
     if (i < ncell && start[i] > start[i + 1]) atomicAdd(bad, 1u);
     if (i == 0 && start[ncell] != (uint32_t)n) atomicAdd(bad, 1u);
+    // between steps the histogram and its tile sums are clear
+    if (i < ncell && count[i] != 0u) atomicAdd(bad, 1u);
+    if (i < ntile && tsum[i] != 0u) atomicAdd(bad, 1u);
 }
 
 extern "C" int rd_check_invariants(rd_system* h, long* violations)
@@ -1924,7 +1934,7 @@ extern "C" int rd_check_invariants(rd_system* h, long* violations)
                                h->d_reps, h->d_waves, h->nwaves, d_seen, d_bad);
         const long m = std::max<long>(h->n, h->ncell);
         hipLaunchKernelGGL(k_check_counts, dim3(std::max(1, nblk(m, RD_BLOCK))), dim3(RD_BLOCK), 0, h->stream, d_seen,
-                           h->n, h->d_start, h->ncell, d_bad);
+                           h->n, h->d_start, h->ncell, h->d_count, h->d_bsum, h->nscan, d_bad);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, h->stream);
