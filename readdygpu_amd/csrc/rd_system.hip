@@ -181,6 +181,22 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ 
 {
     __shared__ uint32_t lds[RD_BLOCK / 64];
     __shared__ uint32_t pre_sh;
+    // this tile's counts are loaded first so that their latency overlaps the
+    // sum over the earlier tiles
+    size_t base = (size_t)blockIdx.x * RD_SCAN_TILE + (size_t)threadIdx.x * RD_SCAN_ITEMS;
+    uint32_t c[RD_SCAN_ITEMS];
+    bool full = base + RD_SCAN_ITEMS <= (size_t)ncell;
+    if (full) {
+        const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
+#pragma unroll
+        for (int k = 0; k < RD_SCAN_ITEMS / 4; ++k) {
+            uint4 q = c4[k];
+            c[4 * k] = q.x; c[4 * k + 1] = q.y; c[4 * k + 2] = q.z; c[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < RD_SCAN_ITEMS; ++k) c[k] = (base + k < (size_t)ncell) ? count[base + k] : 0u;
+    }
     {
         uint32_t acc = 0;
         for (int i = threadIdx.x; i < (int)blockIdx.x; i += RD_BLOCK) acc += bsum[i];
@@ -197,20 +213,6 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ 
         __syncthreads();
     }
     const uint32_t pre = pre_sh;
-    size_t base = (size_t)blockIdx.x * RD_SCAN_TILE + (size_t)threadIdx.x * RD_SCAN_ITEMS;
-    uint32_t c[RD_SCAN_ITEMS];
-    bool full = base + RD_SCAN_ITEMS <= (size_t)ncell;
-    if (full) {
-        const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
-#pragma unroll
-        for (int k = 0; k < RD_SCAN_ITEMS / 4; ++k) {
-            uint4 q = c4[k];
-            c[4 * k] = q.x; c[4 * k + 1] = q.y; c[4 * k + 2] = q.z; c[4 * k + 3] = q.w;
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < RD_SCAN_ITEMS; ++k) c[k] = (base + k < (size_t)ncell) ? count[base + k] : 0u;
-    }
     uint32_t v = 0;
 #pragma unroll
     for (int k = 0; k < RD_SCAN_ITEMS; ++k) v += c[k];
