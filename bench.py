@@ -68,7 +68,26 @@ def cpu_baseline(spec, xyz, types, budget_s=15.0):
         "sample": f"plain_1m, {steps} steps of all {n} particles, FP64 oracle (oracle/rd_oracle.c orc_run), 1 thread",
         "value_all_cores": n * steps / dtn,
         "cores_all": nth,
+        "host": host_cpu(),
     }
+
+
+def host_cpu():
+    """CPU model and core counts of the host the baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
 
 
 def bench_handle(sim, steps, warmup, tune_steps=0):
