@@ -175,7 +175,7 @@ __device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
 // this one function, so all of them produce the same bits.
 template <bool ENERGY>
 __device__ __forceinline__ void pair_add(const RdTab& T, int rowbase, float dx, float dy, float dz, float d2, int tj,
-                                         long long& ax, long long& ay, long long& az, float& e)
+                                         long long& ax, long long& ay, long long& az, double& e)
 {
     if (d2 < T.rc2[rowbase + tj] && d2 > 0.0f) {
         float rinv = __builtin_amdgcn_rsqf(d2);
@@ -189,7 +189,7 @@ __device__ __forceinline__ void pair_add(const RdTab& T, int rowbase, float dx, 
             pre += pair_pre_fast<ENERGY>(T.pot2[pi], r0, r, rinv, &ep);
         }
         ax += to_fix(pre * dx); ay += to_fix(pre * dy); az += to_fix(pre * dz);
-        if (ENERGY) e += 0.5f * ep;
+        if (ENERGY) e += (double)(0.5f * ep);
     }
 }
 
@@ -201,7 +201,7 @@ __device__ __forceinline__ float dist2(float dx, float dy, float dz)
 // the pair (p, q) with the minimum-image separation
 template <bool ENERGY>
 __device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p, float4 q, const RdRep& R,
-                                           long long& ax, long long& ay, long long& az, float& e)
+                                           long long& ax, long long& ay, long long& az, double& e)
 {
     float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
     float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
@@ -216,7 +216,7 @@ __device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p
 // range is rejected before the type-pair table is read.
 template <bool ENERGY>
 __device__ __forceinline__ void pair_inner(const RdTab& T, int rowbase, float rmax2, float4 p, float4 q,
-                                           long long& ax, long long& ay, long long& az, float& e)
+                                           long long& ax, long long& ay, long long& az, double& e)
 {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float d2 = dist2(dx, dy, dz);
@@ -226,7 +226,7 @@ __device__ __forceinline__ void pair_inner(const RdTab& T, int rowbase, float rm
 // one stencil candidate: raw difference for interior homes, minimum image otherwise
 template <bool ENERGY>
 __device__ __forceinline__ void pair_any(const RdTab& T, int rowbase, float rmax2, bool interior, float4 p, float4 q,
-                                         const RdRep& R, long long& ax, long long& ay, long long& az, float& e)
+                                         const RdRep& R, long long& ax, long long& ay, long long& az, double& e)
 {
     if (interior) pair_inner<ENERGY>(T, rowbase, rmax2, p, q, ax, ay, az, e);
     else pair_accum<ENERGY>(T, rowbase, p, q, R, ax, ay, az, e);
@@ -242,7 +242,7 @@ __device__ __forceinline__ bool interior_cell(const RdRep& R, int cx, int cy, in
 template <bool ENERGY>
 __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const RdTab& T,
                                               const RdPot1Dev* __restrict__ pot1, uint32_t s, float4 p, long long ax,
-                                              long long ay, long long az, float* e)
+                                              long long ay, long long az, double* e)
 {
     float gx = (float)((double)ax * RD_FIX_INV);
     float gy = (float)((double)ay * RD_FIX_INV);
@@ -252,14 +252,16 @@ __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const
     if (gext && ((P->ext_mask >> ti) & 1u)) {
         const float4 q = gext[s];
         gx = gx + q.x; gy = gy + q.y; gz = gz + q.z;
-        if (ENERGY) *e += q.w;
+        if (ENERGY) *e += (double)q.w;
     }
     const int np1 = P->n_pot1;
     const float prad = T.prad[ti];
     for (int k = 0; k < np1; ++k) {
         const RdPot1Dev& q = pot1[k];
         if (!((q.mask >> ti) & 1u)) continue;
-        ext_grad<ENERGY>(q, prad, p.x, p.y, p.z, gx, gy, gz, e);
+        float ek = 0.0f;
+        ext_grad<ENERGY>(q, prad, p.x, p.y, p.z, gx, gy, gz, &ek);
+        if (ENERGY) *e += (double)ek;
     }
     return make_float3(gx, gy, gz);
 }
@@ -270,7 +272,7 @@ __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const
 template <bool ENERGY>
 __device__ __forceinline__ void bond_accum(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
                                            const float4* __restrict__ ps, float4 p, long long& ax, long long& ay,
-                                           long long& az, float& e)
+                                           long long& az, double& e)
 {
     const uint32_t gi = (uint32_t)R.off + w_index(p.w);
     const int b1 = P->bstart[gi + 1];
@@ -290,7 +292,7 @@ __device__ __forceinline__ void bond_accum(const RdDev* __restrict__ P, const Rd
         float ep = 0.0f;
         const float pre = pair_pre_fast<ENERGY>(P->bpot[bd.y], r0, r, rinv, &ep);
         ax += to_fix(pre * dx); ay += to_fix(pre * dy); az += to_fix(pre * dz);
-        if (ENERGY) e += 0.5f * ep;
+        if (ENERGY) e += (double)(0.5f * ep);
     }
 }
 
@@ -307,12 +309,12 @@ template <bool ENERGY>
 __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
                                               const RdPot1Dev* __restrict__ pot1,
                                               const float4* __restrict__ ps, const uint32_t* __restrict__ start,
-                                              uint32_t s, float4 p, float* energy, RdHitList* __restrict__ il)
+                                              uint32_t s, float4 p, double* energy, RdHitList* __restrict__ il)
 {
     const int ti = w_type(p.w);
     const int rowbase = ti * P->nt;
     long long ax = 0, ay = 0, az = 0;
-    float e = 0.0f;
+    double e = 0.0;        // pair, bond and external energies accumulate in FP64
     const int Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
     int cx, cy, cz;
     cell_coords(R, p.x, p.y, p.z, cx, cy, cz);

@@ -102,11 +102,11 @@ __global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict
     if (s < (uint32_t)it.s1) {
         const RdRep R = reps[it.rep];
         float4 p = ps[s];
-        float ef = 0.0f;
+        double ef = 0.0;
         float3 g = gather_grad<true>(P, R, T, pot1, ps, start, s, p, &ef, &il[threadIdx.x >> 6]);
         size_t gi = (size_t)R.off + w_index(p.w);
         grad[3 * gi] = g.x; grad[3 * gi + 1] = g.y; grad[3 * gi + 2] = g.z;
-        e = (double)ef;
+        e = ef;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
