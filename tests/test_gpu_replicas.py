@@ -204,3 +204,43 @@ def test_replica_reactions_equal_separate_runs(batch):
     with Simulation(spec, xyz, types, replicas=(boxes, counts)) as sim:
         with pytest.raises(RdError):             # a pure creation needs its replica
             sim.apply_reactions(np.array([1, 0, 1, 1.0, 1.0, 1.0, -1], np.float32))
+
+
+def test_ragged_replicas_with_empty_and_wave_boundary_sizes(batch):
+    """Replicas of 0, 1, 63, 64, 65, 0 and 130 particles (empty replicas, one
+    particle, sizes on both sides of a 64-lane wave): each evolves bit for bit
+    as the same box alone, and the pair sets and gradients match the oracle."""
+    spec, boxes, _, _, _ = batch
+    counts = np.array([0, 1, 63, 64, 65, 0, 130], dtype=np.int32)
+    boxes = boxes[:len(counts)]
+    rng = np.random.default_rng(11)
+    xyz, types = [], []
+    for r, c in enumerate(counts):
+        b = boxes[r]
+        lo, hi = np.array([b[0], b[2], b[4]]), np.array([b[1], b[3], b[5]])
+        xyz.append((lo + rng.random((c, 3)) * (hi - lo) * 0.999).astype(np.float32))
+        types.append(rng.integers(0, spec.n_types, c).astype(np.int32))
+    xyz, types = np.concatenate(xyz), np.concatenate(types)
+    with Simulation(spec, xyz, types, replicas=(boxes, counts)) as sim:
+        sim.simulate(9)
+        xb, tb = sim.copy_pos_from_device()
+        g, e = sim.gradients()
+        p = sim.neighbor_pairs()
+    pref, eref = [], 0.0
+    for r in range(len(counts)):
+        s, x, t, off = replica(spec, boxes, counts, xyz, types, r)
+        n = len(t)
+        with Simulation(s, x, t) as one:
+            one.simulate(9)
+            xr, tr = one.copy_pos_from_device()
+        assert np.array_equal(xb[off:off + n], xr), f"replica {r}"
+        assert np.array_equal(tb[off:off + n], tr)
+        if n == 0:
+            continue
+        o = orc.OracleSystem(s)
+        gr, er = o.gradients(xb[off:off + n].astype(np.float64), tb[off:off + n], brute=True)
+        assert np.abs(g[off:off + n] - gr).max() <= 2e-5 * np.abs(gr).max() + 1e-5, f"replica {r}"
+        eref += er
+        pref.append(o.pairs(xb[off:off + n], tb[off:off + n], brute=True) + off)
+    assert abs(e - eref) <= 1e-5 * max(1.0, abs(eref))
+    assert np.array_equal(p, np.concatenate(pref))
