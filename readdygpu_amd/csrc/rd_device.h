@@ -114,10 +114,11 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        uint32_t lo0 = 0xD2511F53u * c[0];
-        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
-        uint32_t lo1 = 0xCD9E8D57u * c[2];
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        // one 32x32->64 multiply per word gives both halves
+        const uint64_t m0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t m1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t lo0 = (uint32_t)m0, hi0 = (uint32_t)(m0 >> 32);
+        const uint32_t lo1 = (uint32_t)m1, hi1 = (uint32_t)(m1 >> 32);
         uint32_t n0 = hi1 ^ c[1] ^ k0;
         uint32_t n2 = hi0 ^ c[3] ^ k1;
         c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
