@@ -59,8 +59,7 @@ struct rd_system {
     // device buffers
     float4* d_ps = nullptr;
     float4* d_pn = nullptr;
-    uint32_t* d_key = nullptr;
-    uint32_t* d_rank = nullptr;
+    uint2* d_kr = nullptr;        // (next cell, rank inside it) of each unsorted slot
     uint32_t* d_count = nullptr;
     uint32_t* d_start = nullptr;
     uint32_t* d_bsum = nullptr;

@@ -3,9 +3,9 @@
 // Data layout in HBM (all replicas of a handle; a plain system is one):
 //   ps[N]   float4, sorted by cell: x, y, z, w = (local index << 6) | type
 //   pn[N]   float4, positions after the update, in the slot order of ps
-//   key[N]  next global cell of slot s;  rank[N]: its rank inside the cell
+//   kr[N]   uint2: next global cell of slot s and its rank inside that cell
 //   count[ncell], start[ncell+1]: counting-sort histogram and offsets over the
-//   concatenated cell ranges of all replicas
+//   concatenated cell ranges of all replicas; tsum[]: per-2048-cell tile sums
 // One step = force/update (gather forces, Euler-Maruyama, wrap, next cell key,
 // histogram) -> scan -> scatter.  Replaces orderOne, warpOrderTwo and update
 // (CudaSimulation.cu:2309-2476) and the linked-list cell maintenance with
@@ -411,7 +411,7 @@ __device__ __forceinline__ float wrap_coord(float x, float lo, float hi, float L
 __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
                                                 RdStepState* __restrict__ st, const float* __restrict__ noise,
                                                 int ntot, uint32_t s, float4 p, float3 g, float4* __restrict__ pn,
-                                                uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                uint2* __restrict__ kr,
                                                 uint32_t* __restrict__ count, uint32_t* __restrict__ tsum)
 {
     const int t = w_type(p.w);
@@ -441,8 +441,7 @@ __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, con
     pn[s] = make_float4(x, y, zc, p.w);
     if (P->gid_n) P->gid_n[s] = (uint32_t)R.off + idx;
     uint32_t k = cell_key(R, x, y, zc);
-    key[s] = k;
-    rank[s] = cell_histogram(count, tsum, k);
+    kr[s] = make_uint2(k, cell_histogram(count, tsum, k));
 }
 
 // the wave's work item: replica and slot range (waves never span replicas)

@@ -22,7 +22,7 @@
 __global__ __launch_bounds__(RD_BLOCK) void k_pack(const float* __restrict__ xyz, const int* __restrict__ type,
                                                    const int* __restrict__ rep_of, int n,
                                                    const RdRep* __restrict__ reps, float4* __restrict__ pn,
-                                                   uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                   uint2* __restrict__ kr,
                                                    uint32_t* __restrict__ count, uint32_t* __restrict__ tsum,
                                                    uint32_t* __restrict__ gid_n)
 {
@@ -33,8 +33,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pack(const float* __restrict__ xyz
     pn[i] = make_float4(x, y, z, w_pack((uint32_t)(i - R.off), type[i]));
     if (gid_n) gid_n[i] = (uint32_t)i;
     uint32_t k = cell_key(R, x, y, z);
-    key[i] = k;
-    rank[i] = cell_histogram(count, tsum, k);
+    kr[i] = make_uint2(k, cell_histogram(count, tsum, k));
 }
 
 // wave index and lane of this thread; blocks of RD_BLOCK = 4 waves
@@ -48,7 +47,7 @@ __device__ __forceinline__ int wave_id() { return xcd_block(blockIdx.x, gridDim.
 // the type-pair table is staged in LDS once per workgroup.
 template <int BLOCK, int PPT>
 __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict__ ps, float4* __restrict__ pn,
-                                                           uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
+                                                           uint2* __restrict__ kr,
                                                            uint32_t* __restrict__ count, uint32_t* __restrict__ tsum,
                                                            const uint32_t* __restrict__ start,
                                                            const RdDev* __restrict__ P,
@@ -69,7 +68,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict
         const RdRep R = reps[it.rep];
         float4 p = ps[s];
         float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
-        em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count, tsum);
+        em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, kr, count, tsum);
     } else {
         for (int k = 0; k < PPT; ++k) {
             const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w + k));
@@ -78,7 +77,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict
                 const RdRep R = reps[it.rep];
                 float4 p = ps[s];
                 float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
-                em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, key, rank, count, tsum);
+                em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, kr, count, tsum);
             }
         }
     }
@@ -239,8 +238,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ 
 }
 
 // counting-sort scatter; advances the step counter when `advance` is set
-__global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__ pn, const uint32_t* __restrict__ key,
-                                                      const uint32_t* __restrict__ rank,
+__global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__ pn, const uint2* __restrict__ kr,
                                                       const uint32_t* __restrict__ start, float4* __restrict__ ps,
                                                       int n, RdStepState* __restrict__ st, int advance,
                                                       const uint32_t* __restrict__ gid_n, uint32_t* __restrict__ slot_of,
@@ -250,7 +248,8 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__
     for (int t = s; t < ntile; t += gridDim.x * RD_BLOCK) tsum[t] = 0u;
     if (advance && s == 0) st->step += 1ull;
     if (s >= n) return;
-    uint32_t dst = start[key[s]] + rank[s];
+    const uint2 q = kr[s];
+    const uint32_t dst = start[q.x] + q.y;
     ps[dst] = pn[s];
     if (slot_of) slot_of[gid_n[s]] = dst;
 }
@@ -526,13 +525,14 @@ static void free_dev(rd_system* h)
 {
     free_bonds(h);
     free_frames(h);
-    void* ptrs[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_start, h->d_bsum,
+    void* ptrs[] = { h->d_ps, h->d_pn, h->d_kr, h->d_count, h->d_start, h->d_bsum,
                      h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_pairs,
                      h->d_xyz_io, h->d_type_io, h->d_etot, h->d_reps, h->d_waves, h->d_rep_of };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->d_ps = h->d_pn = nullptr;
-    h->d_key = h->d_rank = h->d_count = h->d_start = h->d_bsum = nullptr;
+    h->d_kr = nullptr;
+    h->d_count = h->d_start = h->d_bsum = nullptr;
     h->d_P = nullptr; h->d_pot1 = nullptr; h->d_st = nullptr; h->d_noise = nullptr;
     h->d_rdf = nullptr; h->d_pairs = nullptr;
     h->d_xyz_io = nullptr; h->d_type_io = nullptr; h->d_etot = nullptr;
@@ -885,15 +885,14 @@ static int grow_particles(rd_system* h, int need, bool keep_ps)
         HIPCK(h, hipStreamSynchronize(h->stream));
     }
     {
-        void* ps[] = { h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_xyz_io, h->d_type_io, h->d_rep_of };
+        void* ps[] = { h->d_ps, h->d_pn, h->d_kr, h->d_xyz_io, h->d_type_io, h->d_rep_of };
         for (void* p : ps)
             if (p) (void)hipFree(p);
-        h->d_pn = nullptr; h->d_key = h->d_rank = nullptr; h->d_xyz_io = nullptr; h->d_type_io = nullptr;
+        h->d_pn = nullptr; h->d_kr = nullptr; h->d_xyz_io = nullptr; h->d_type_io = nullptr;
         h->d_rep_of = nullptr;
         h->d_ps = nps;
         HIPCK(h, hipMalloc(&h->d_pn, sizeof(float4) * cap));
-        HIPCK(h, hipMalloc(&h->d_key, sizeof(uint32_t) * cap));
-        HIPCK(h, hipMalloc(&h->d_rank, sizeof(uint32_t) * cap));
+        HIPCK(h, hipMalloc(&h->d_kr, sizeof(uint2) * cap));
         HIPCK(h, hipMalloc(&h->d_xyz_io, sizeof(float) * 3 * cap));
         HIPCK(h, hipMalloc(&h->d_type_io, sizeof(int) * cap));
         HIPCK(h, hipMalloc(&h->d_rep_of, sizeof(int) * cap));
@@ -955,7 +954,7 @@ static void launch_sort(rd_system* h, int advance)
     hipLaunchKernelGGL(k_scan_apply, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum,
                        h->d_start);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(h->n, 1), RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_pn,
-                       h->d_key, h->d_rank, h->d_start, h->d_ps, h->n, h->d_st, advance,
+                       h->d_kr, h->d_start, h->d_ps, h->n, h->d_st, advance,
                        (const uint32_t*)h->d_gid, h->d_slot, h->d_bsum, h->nscan);
 }
 
@@ -965,7 +964,7 @@ template <int B, int PPT>
 static void launch_force(rd_system* h)
 {
     hipLaunchKernelGGL((k_force_update<B, PPT>), dim3(std::max(1, nblk(h->nwaves, B / 64 * PPT))), dim3(B), 0,
-                       h->stream, h->d_ps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_bsum, h->d_start, h->d_P,
+                       h->stream, h->d_ps, h->d_pn, h->d_kr, h->d_count, h->d_bsum, h->d_start, h->d_P,
                        h->d_reps,
                        h->d_waves, h->nwaves, h->d_pot1, h->d_st, h->d_noise, h->n);
 }
@@ -1097,7 +1096,7 @@ static int upload(rd_system* h)
                                     h->stream));
         }
         hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io, h->d_type_io,
-                           h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count, h->d_bsum, h->d_gid);
+                           h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_kr, h->d_count, h->d_bsum, h->d_gid);
     }
     launch_sort(h, 0);
     HIPCK(h, hipGetLastError());
@@ -1743,7 +1742,7 @@ extern "C" int rd_frame_rewind(rd_system* h)
     // captured step graphs stay valid)
     if (h->n > 0)
         hipLaunchKernelGGL(k_pack, dim3(nblk(h->n, RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_xyz_io,
-                           h->d_type_io, h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_key, h->d_rank, h->d_count,
+                           h->d_type_io, h->d_rep_of, h->n, h->d_reps, h->d_pn, h->d_kr, h->d_count,
                            h->d_bsum, h->d_gid);
     launch_sort(h, 0);
     HIPCK(h, hipGetLastError());
