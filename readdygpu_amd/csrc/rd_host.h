@@ -110,6 +110,8 @@ struct rd_system {
 
     // RDF
     unsigned long long* d_rdf = nullptr;
+    unsigned long long* d_rdf_part = nullptr;   // rdf_ncopy copies of d_rdf (cleared between calls)
+    int rdf_ncopy = 0;
     double rdf_rmax = 0.0;
     int rdf_bins = 0;
     long rdf_frames = 0;

@@ -358,62 +358,138 @@ __global__ __launch_bounds__(RD_BLOCK) void k_noise_fill(const RdDev* __restrict
     out[base] = z.x; out[base + ntot] = z.y; out[base + 2 * (size_t)ntot] = z.z;
 }
 
-// RDF histogram, privatised per wave in LDS (CudaSimulation.cu:1784-1827 counts
-// every ordered pair once per side; here each slot counts its own partners).
+// RDF histogram (CudaSimulation.cu:1784-1827 counts every ordered pair once
+// per side; here each slot counts its own partners).
 // counts[((rep*nt + ti)*nt + tj)*nbins + bin]
-template <bool LDS>
-__global__ __launch_bounds__(RD_BLOCK) void k_rdf(const float4* __restrict__ ps, const uint32_t* __restrict__ start,
-                                                  const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
-                                                  const RdWave* __restrict__ waves, int nwaves, int S, float rmax,
-                                                  int nbins, unsigned long long* __restrict__ counts)
+// Candidates with d2 >= rcut2 are skipped before the square root: rcut2 is
+// rmax^2 rounded up by 1e-6 relative, and sqrtf is correctly rounded, so every
+// skipped candidate has sqrtf(d2) >= rmax and the counted set is unchanged.
+// `wrap` false: the home cell's stencil neither wraps nor touches cell 0
+// (which also holds the partial slab at the top of the box), and it spans
+// less than half the box, so every raw difference is already its minimum
+// image (min_image would return it unchanged).
+__device__ __forceinline__ int rdf_bin(const RdRep& R, float4 p, float4 q, int ti, int nt, float rmax, float rcut2,
+                                       int nbins, bool wrap)
 {
-    extern __shared__ uint32_t hist_all[];
+    float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    if (wrap) {
+        dx = min_image(dx, R.Lb[0], R.half[0]);
+        dy = min_image(dy, R.Lb[1], R.half[1]);
+        dz = min_image(dz, R.Lb[2], R.half[2]);
+    }
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    if (!(d2 < rcut2)) return -1;
+    float r = sqrtf(d2);
+    if (!(r < rmax)) return -1;
+    int bin = (int)floorf((r * (float)nbins) / rmax);
+    if (bin > nbins - 1) bin = nbins - 1;
+    return (ti * nt + w_type(q.w)) * nbins + bin;
+}
+
+__device__ __forceinline__ bool rdf_wraps(const RdRep& R, int cx, int cy, int cz, int S)
+{
+    const int L[3] = { R.L[0], R.L[1], R.L[2] }, c[3] = { cx, cy, cz };
+    for (int d = 0; d < 3; ++d)
+        if (c[d] < S + 1 || c[d] > L[d] - 1 - S || (float)(S + 2) * R.cell >= R.half[d]) return true;
+    return false;
+}
+
+// One LDS histogram per workgroup, shared by its waves when their work items
+// belong to one replica (a workgroup straddling two replicas adds to the
+// global counts directly).  Workgroup b adds its histogram to copy b % ncopy
+// of the counts; k_rdf_fold sums the copies, so the 64-bit atomics of
+// thousands of workgroups do not queue on the same few thousand addresses.
+__global__ __launch_bounds__(RD_BLOCK) void k_rdf_lds(const float4* __restrict__ ps, const uint32_t* __restrict__ start,
+                                                      const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
+                                                      const RdWave* __restrict__ waves, int nwaves, int S, float rmax,
+                                                      float rcut2, int nbins, unsigned long long* __restrict__ part,
+                                                      long part_stride, int ncopy,
+                                                      unsigned long long* __restrict__ counts)
+{
+    extern __shared__ uint32_t hist[];
+    __shared__ int rep_sh[RD_BLOCK / 64];
     const int nt = P->nt;
     const int nh = nt * nt * nbins;
-    const int wv = threadIdx.x >> 6;
-    uint32_t* hist = hist_all + (size_t)wv * nh;
-    if (LDS) {
-        for (int k = threadIdx.x; k < nh * (RD_BLOCK / 64); k += RD_BLOCK) hist_all[k] = 0u;
-        __syncthreads();
+    const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(wave_id()));
+    if ((threadIdx.x & 63) == 0) rep_sh[threadIdx.x >> 6] = it.s1 > it.s0 ? it.rep : -1;
+    for (int k = threadIdx.x; k < nh; k += RD_BLOCK) hist[k] = 0u;
+    __syncthreads();
+    int rep0 = -1;
+    bool one = true;
+    for (int v = 0; v < RD_BLOCK / 64; ++v) {
+        const int r = rep_sh[v];
+        if (r < 0) continue;
+        if (rep0 < 0) rep0 = r;
+        else if (r != rep0) one = false;
     }
-    const RdWave it = wave_item(waves, nwaves, wave_id());
     const int s = it.s0 + (threadIdx.x & 63);
-    const RdRep R = reps[it.rep];
-    unsigned long long* cnt = counts + (size_t)it.rep * nh;
     if (s < it.s1) {
-        float4 p = ps[s];
-        int ti = w_type(p.w);
+        const RdRep R = reps[it.rep];
+        unsigned long long* cnt = counts + (size_t)it.rep * nh;
+        const float4 p = ps[s];
+        const int ti = w_type(p.w);
         int cx, cy, cz;
         cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
-        const float fnb = (float)nbins;
+        const bool wrap = rdf_wraps(R, cx, cy, cz, S);
         for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
             if ((int)j == s) return;
-            float4 q = ps[j];
-            float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
-            float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
-            float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
-            float d2 = dx * dx;
-            d2 = d2 + dy * dy;
-            d2 = d2 + dz * dz;
-            float r = sqrtf(d2);
-            if (r < rmax) {
-                int bin = (int)floorf((r * fnb) / rmax);
-                if (bin > nbins - 1) bin = nbins - 1;
-                int h = (ti * nt + w_type(q.w)) * nbins + bin;
-                if (LDS) atomicAdd(&hist[h], 1u);
-                else atomicAdd(&cnt[h], 1ull);
-            }
+            const int h = rdf_bin(R, p, ps[j], ti, nt, rmax, rcut2, nbins, wrap);
+            if (h < 0) return;
+            if (one) atomicAdd(&hist[h], 1u);
+            else atomicAdd(&cnt[h], 1ull);
         });
     }
-    if (LDS) {
-        __syncthreads();
-        if (it.s1 > it.s0) {
-            for (int k = threadIdx.x & 63; k < nh; k += 64) {
-                uint32_t v = hist[k];
-                if (v) atomicAdd(&cnt[k], (unsigned long long)v);
-            }
+    __syncthreads();
+    if (one && rep0 >= 0) {
+        unsigned long long* cnt = part + (size_t)(blockIdx.x % ncopy) * (size_t)part_stride + (size_t)rep0 * nh;
+        for (int k = threadIdx.x; k < nh; k += RD_BLOCK) {
+            const uint32_t v = hist[k];
+            if (v) atomicAdd(&cnt[k], (unsigned long long)v);
         }
     }
+}
+
+// counts += the sum of the copies; the copies are cleared for the next call
+__global__ __launch_bounds__(RD_BLOCK) void k_rdf_fold(unsigned long long* __restrict__ part, long n, int ncopy,
+                                                       unsigned long long* __restrict__ counts)
+{
+    const long i = (long)blockIdx.x * RD_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long v = 0;
+    for (int c = 0; c < ncopy; ++c) {
+        v += part[(size_t)c * n + i];
+        part[(size_t)c * n + i] = 0ull;
+    }
+    counts[i] += v;
+}
+
+// histogram too large for LDS: direct 64-bit atomics on the global counts
+__global__ __launch_bounds__(RD_BLOCK) void k_rdf_global(const float4* __restrict__ ps,
+                                                         const uint32_t* __restrict__ start,
+                                                         const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
+                                                         const RdWave* __restrict__ waves, int nwaves, int S,
+                                                         float rmax, float rcut2, int nbins,
+                                                         unsigned long long* __restrict__ counts)
+{
+    const int nt = P->nt;
+    const int nh = nt * nt * nbins;
+    const RdWave it = wave_item(waves, nwaves, wave_id());
+    const int s = it.s0 + (threadIdx.x & 63);
+    if (s >= it.s1) return;
+    const RdRep R = reps[it.rep];
+    unsigned long long* cnt = counts + (size_t)it.rep * nh;
+    const float4 p = ps[s];
+    const int ti = w_type(p.w);
+    int cx, cy, cz;
+    cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
+    const bool wrap = rdf_wraps(R, cx, cy, cz, S);
+    for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
+        if ((int)j == s) return;
+        const int h = rdf_bin(R, p, ps[j], ti, nt, rmax, rcut2, nbins, wrap);
+        if (h >= 0) atomicAdd(&cnt[h], 1ull);
+    });
 }
 
 // pairs (i<j by index) with d2 < thr2[ti][tj]; global particle indices
@@ -526,7 +602,7 @@ static void free_dev(rd_system* h)
     free_bonds(h);
     free_frames(h);
     void* ptrs[] = { h->d_ps, h->d_pn, h->d_kr, h->d_count, h->d_start, h->d_bsum,
-                     h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_pairs,
+                     h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_rdf_part, h->d_pairs,
                      h->d_xyz_io, h->d_type_io, h->d_etot, h->d_reps, h->d_waves, h->d_rep_of };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -534,7 +610,7 @@ static void free_dev(rd_system* h)
     h->d_kr = nullptr;
     h->d_count = h->d_start = h->d_bsum = nullptr;
     h->d_P = nullptr; h->d_pot1 = nullptr; h->d_st = nullptr; h->d_noise = nullptr;
-    h->d_rdf = nullptr; h->d_pairs = nullptr;
+    h->d_rdf = nullptr; h->d_rdf_part = nullptr; h->d_pairs = nullptr;
     h->d_xyz_io = nullptr; h->d_type_io = nullptr; h->d_etot = nullptr;
     h->d_reps = nullptr; h->d_waves = nullptr; h->d_rep_of = nullptr;
     h->cap = 0; h->ncell_alloc = 0; h->pairs_cap = 0; h->noise_cap = 0; h->nrep_alloc = 0; h->nwave_alloc = 0;
@@ -1350,9 +1426,18 @@ extern "C" int rd_rdf_configure(rd_system* h, double rmax, int nbins)
     h->rdf_nrep = (int)h->reps.size();
     size_t nh = (size_t)h->rdf_nrep * h->nt * h->nt * nbins;
     if (h->d_rdf) (void)hipFree(h->d_rdf);
+    if (h->d_rdf_part) (void)hipFree(h->d_rdf_part);
     h->d_rdf = nullptr;
+    h->d_rdf_part = nullptr;
     HIPCK(h, hipMalloc(&h->d_rdf, sizeof(unsigned long long) * nh));
     HIPCK(h, hipMemsetAsync(h->d_rdf, 0, sizeof(unsigned long long) * nh, h->stream));
+    // copies of the counts the LDS histograms are flushed into (k_rdf_lds),
+    // 16 while they stay within 64 MB
+    h->rdf_ncopy = 16;
+    while (h->rdf_ncopy > 1 && sizeof(unsigned long long) * nh * (size_t)h->rdf_ncopy > ((size_t)64 << 20))
+        h->rdf_ncopy >>= 1;
+    HIPCK(h, hipMalloc(&h->d_rdf_part, sizeof(unsigned long long) * nh * (size_t)h->rdf_ncopy));
+    HIPCK(h, hipMemsetAsync(h->d_rdf_part, 0, sizeof(unsigned long long) * nh * (size_t)h->rdf_ncopy, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
     h->rdf_frames = 0;
     h->rdf_pairs_sum.assign((size_t)h->rdf_nrep * h->nt * h->nt, 0.0);
@@ -1377,16 +1462,23 @@ extern "C" int rd_rdf_calculate(rd_system* h)
     if ((int)h->rep_dev.size() != h->rdf_nrep) return set_err(h, RD_ESTATE, "replicas changed: reconfigure the RDF");
     int S = std::max(1, (int)std::ceil(h->rdf_rmax * 1.0001 / h->cell));
     size_t nh = (size_t)h->nt * h->nt * h->rdf_bins;
-    size_t lds = nh * sizeof(uint32_t) * (RD_BLOCK / 64);
+    // one histogram per workgroup of up to 160 KB
+    size_t lds = nh * sizeof(uint32_t);
+    const float rmax = (float)h->rdf_rmax;
+    const float rcut2 = (float)((double)rmax * (double)rmax * (1.0 + 1e-6));
     if (h->n > 0) {
-        if (lds <= 160 * 1024) {
+        if (lds <= 160 * 1024 && h->d_rdf_part) {
             if (lds > 64 * 1024)
-                HIPCK(h, hipFuncSetAttribute((const void*)k_rdf<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(k_rdf<true>, dim3(wave_blocks(h)), dim3(RD_BLOCK), lds, h->stream, h->d_ps, h->d_start,
-                               h->d_P, h->d_reps, h->d_waves, h->nwaves, S, (float)h->rdf_rmax, h->rdf_bins, h->d_rdf);
+                HIPCK(h, hipFuncSetAttribute((const void*)k_rdf_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            const long total = (long)h->rdf_nrep * (long)nh;
+            hipLaunchKernelGGL(k_rdf_lds, dim3(wave_blocks(h)), dim3(RD_BLOCK), lds, h->stream, h->d_ps, h->d_start,
+                               h->d_P, h->d_reps, h->d_waves, h->nwaves, S, rmax, rcut2, h->rdf_bins, h->d_rdf_part,
+                               total, h->rdf_ncopy, h->d_rdf);
+            hipLaunchKernelGGL(k_rdf_fold, dim3(std::max(1, nblk(total, RD_BLOCK))), dim3(RD_BLOCK), 0, h->stream,
+                               h->d_rdf_part, total, h->rdf_ncopy, h->d_rdf);
         } else {
-            hipLaunchKernelGGL(k_rdf<false>, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
-                               h->d_P, h->d_reps, h->d_waves, h->nwaves, S, (float)h->rdf_rmax, h->rdf_bins, h->d_rdf);
+            hipLaunchKernelGGL(k_rdf_global, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
+                               h->d_P, h->d_reps, h->d_waves, h->nwaves, S, rmax, rcut2, h->rdf_bins, h->d_rdf);
         }
         HIPCK(h, hipGetLastError());
     }

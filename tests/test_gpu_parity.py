@@ -227,6 +227,33 @@ def test_rdf_counts_bitexact_and_normalised(c1):
     assert abs(g[:, :, 100:].mean() - 1.0) < 0.05     # dilute fluid: g -> 1 at range
 
 
+@pytest.mark.parametrize("cell", [7.0, 6.5])
+def test_rdf_counts_with_a_partial_top_cell(cell):
+    """Lattice edges that leave a partial slab at the top of the box (it wraps
+    into cell 0): particles crowded into that slab and into the cells whose
+    stencils reach cell 0 or stop just short of it give the oracle's counts
+    bit for bit, also where the kernel skips the minimum image."""
+    box = 100.0
+    spec = SystemSpec(box_lo=(0, 0, 0), box_hi=(box, box, box), dt=0.1, seed=5, cell_size=cell)
+    spec.types = [ParticleType("A", 1.0, 1.5), ParticleType("B", 0.5, 2.0)]
+    spec.pot2 = [OrderTwoPotential(HARMONIC, REPULSIVE, 10.0)]
+    rng = np.random.default_rng(21)
+    L = int(box / cell)
+    parts = [rng.random((3000, 3)) * box]
+    for lo_, hi_ in ((L * cell, box), (2 * cell, 5 * cell), (0.0, cell)):
+        q = rng.random((600, 3)) * box
+        q[:, rng.integers(0, 3)] = lo_ + rng.random(600) * (hi_ - lo_)
+        parts.append(q)
+    xyz = np.clip(np.concatenate(parts), 0.0, np.nextafter(box, 0.0)).astype(np.float32)
+    types = rng.integers(0, 2, len(xyz)).astype(np.int32)
+    o = orc.OracleSystem(spec)
+    with Simulation(spec, xyz, types, rdf=(20.0, 100)) as sim:
+        assert sim.info().lattice[0] == L
+        sim.call_rdf_calculation()
+        c, _ = sim.copy_rdf_matrix()
+    assert np.array_equal(c, o.rdf_counts(xyz, types, 20.0, 100, cells=False))
+
+
 @pytest.mark.parametrize("box,cell,n", [(12.0, 4.2, 400), (9.0, 4.0, 200), (30.0, 9.5, 2000)])
 def test_small_lattices_and_wide_cells(box, cell, n):
     """L < 3 (each cell once) and cells wider than the range."""
