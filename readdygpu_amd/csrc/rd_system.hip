@@ -388,7 +388,7 @@ __device__ __forceinline__ int rdf_bin(const RdRep& R, float4 p, float4 q, int t
     return (ti * nt + w_type(q.w)) * nbins + bin;
 }
 
-__device__ __forceinline__ bool rdf_wraps(const RdRep& R, int cx, int cy, int cz, int S)
+__device__ __forceinline__ bool stencil_wraps(const RdRep& R, int cx, int cy, int cz, int S)
 {
     const int L[3] = { R.L[0], R.L[1], R.L[2] }, c[3] = { cx, cy, cz };
     for (int d = 0; d < 3; ++d)
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf_lds(const float4* __restrict__
         const int ti = w_type(p.w);
         int cx, cy, cz;
         cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
-        const bool wrap = rdf_wraps(R, cx, cy, cz, S);
+        const bool wrap = stencil_wraps(R, cx, cy, cz, S);
         for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
             if ((int)j == s) return;
             const int h = rdf_bin(R, p, ps[j], ti, nt, rmax, rcut2, nbins, wrap);
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf_global(const float4* __restric
     const int ti = w_type(p.w);
     int cx, cy, cz;
     cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
-    const bool wrap = rdf_wraps(R, cx, cy, cz, S);
+    const bool wrap = stencil_wraps(R, cx, cy, cz, S);
     for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
         if ((int)j == s) return;
         const int h = rdf_bin(R, p, ps[j], ti, nt, rmax, rcut2, nbins, wrap);
@@ -517,14 +517,18 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ p
     int cx, cy, cz;
     cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
     const uint32_t o = EMIT ? cnt_off[gi] : 0u;
+    const bool wrap = stencil_wraps(R, cx, cy, cz, S);
     uint32_t c = 0;
     for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
         float4 q = ps[j];
         uint32_t jj = w_index(q.w);
         if (jj <= ii) return;
-        float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
-        float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
-        float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
+        float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        if (wrap) {
+            dx = min_image(dx, R.Lb[0], R.half[0]);
+            dy = min_image(dy, R.Lb[1], R.half[1]);
+            dz = min_image(dz, R.Lb[2], R.half[2]);
+        }
         float d2 = dx * dx;
         d2 = d2 + dy * dy;
         d2 = d2 + dz * dz;
