@@ -1471,7 +1471,7 @@ extern "C" int rd_rdf_calculate(rd_system* h)
     const float rmax = (float)h->rdf_rmax;
     const float rcut2 = (float)((double)rmax * (double)rmax * (1.0 + 1e-6));
     if (h->n > 0) {
-        if (lds <= 160 * 1024 && h->d_rdf_part) {
+        if (lds + 64 <= 160 * 1024 && h->d_rdf_part) {          // + the static replica table
             if (lds > 64 * 1024)
                 HIPCK(h, hipFuncSetAttribute((const void*)k_rdf_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             const long total = (long)h->rdf_nrep * (long)nh;
