@@ -254,6 +254,27 @@ def test_rdf_counts_with_a_partial_top_cell(cell):
     assert np.array_equal(c, o.rdf_counts(xyz, types, 20.0, 100, cells=False))
 
 
+@pytest.mark.parametrize("nt,nbins", [(12, 200), (16, 170)])
+def test_rdf_counts_with_large_histograms(nt, nbins):
+    """Histograms of 115-170 KB: the LDS path with a raised LDS limit, or the
+    direct global-atomic path past 160 KB; counts bit for bit the oracle's."""
+    box = 40.0
+    spec = SystemSpec(box_lo=(0, 0, 0), box_hi=(box, box, box), dt=0.1, seed=8)
+    spec.types = [ParticleType(f"T{k}", 1.0, 0.5 + 0.05 * k) for k in range(nt)]
+    spec.pot2 = [OrderTwoPotential(HARMONIC, REPULSIVE, 10.0)]
+    rng = np.random.default_rng(nt)
+    xyz = (rng.random((2500, 3)) * box).astype(np.float32)
+    types = rng.integers(0, nt, len(xyz)).astype(np.int32)
+    o = orc.OracleSystem(spec)
+    with Simulation(spec, xyz, types, rdf=(10.0, nbins)) as sim:
+        sim.simulate(3)
+        x, t = sim.copy_pos_from_device()
+        sim.call_rdf_calculation()
+        c, _ = sim.copy_rdf_matrix()
+    assert c.shape == (nt, nt, nbins)
+    assert np.array_equal(c, o.rdf_counts(x, t, 10.0, nbins, cells=False))
+
+
 @pytest.mark.parametrize("box,cell,n", [(12.0, 4.2, 400), (9.0, 4.0, 200), (30.0, 9.5, 2000)])
 def test_small_lattices_and_wide_cells(box, cell, n):
     """L < 3 (each cell once) and cells wider than the range."""
