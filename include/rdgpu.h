@@ -359,9 +359,11 @@ int rd_get_launch_config(rd_system* h, rd_launch_config* cfg);
 
 /* Autotuner: times `trial_steps` steps for each candidate cell size (factors
  * of the interaction range) and force-kernel workgroup size, then particles
- * per lane at the winning pair; keeps the fastest, and restores the particle
- * state and step counter, so tuning does not change the run.  Values fixed
- * by rd_set_launch_config (block, ppt) or the cell size are kept. */
+ * per lane at the winning pair; keeps the fastest (the default shape, 256
+ * threads and one particle per lane, unless another is more than 2% faster),
+ * and restores the particle state and step counter, so tuning does not change
+ * the run.  Values fixed by rd_set_launch_config (block, ppt) or the cell size
+ * are kept. */
 int rd_autotune(rd_system* h, int trial_steps, double* best_ms_per_step);
 
 /* Lattice cell edge of a running system (<= 0: 1.01 x the largest interaction

@@ -344,32 +344,48 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                 }
             };
             // 27-cell stencil as 9 rows of at most two runs of contiguous
-            // slots.  All 36 run bounds are loaded first (independent loads,
-            // in flight together); then the runs are walked.
+            // slots (two only for homes at the x edge of the lattice).  The
+            // 18 bounds of the first runs are loaded first (independent
+            // loads, in flight together), then those runs are walked; the
+            // second runs follow only in waves that have any, so that their
+            // bounds do not hold registers in the common case.
             int a0, b0, a1, b1;
             const int nr = x_runs(cx, 1, Lx, &a0, &b0, &a1, &b1);
-            uint32_t sb[18], se[18];
-#pragma unroll
-            for (int r = 0; r < 9; ++r) {
+            auto row_of = [&](int r) {
                 int yy = cy + (r % 3) - 1, zz = cz + (r / 3) - 1;
                 yy += yy < 0 ? Ly : 0; yy -= yy >= Ly ? Ly : 0;
                 zz += zz < 0 ? Lz : 0; zz -= zz >= Lz ? Lz : 0;
-                const int row = R.coff + (zz * Ly + yy) * Lx;
-                sb[2 * r] = start[row + a0];
-                se[2 * r] = start[row + b0 + 1];
-                sb[2 * r + 1] = nr == 2 ? start[row + a1] : 0u;
-                se[2 * r + 1] = nr == 2 ? start[row + b1 + 1] : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < 18; ++k) {
-                uint32_t j = sb[k];
-                const uint32_t j1 = se[k];
+                return R.coff + (zz * Ly + yy) * Lx;
+            };
+            auto walk = [&](uint32_t j, uint32_t j1) {
                 for (; j + 1 < j1; j += 2) {
                     const float4 q0 = ps[j], q1 = ps[j + 1];
                     test(j, q0);
                     test(j + 1, q1);
                 }
                 if (j < j1) test(j, ps[j]);
+            };
+            {
+                uint32_t sb[9], se[9];
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    const int row = row_of(r);
+                    sb[r] = start[row + a0];
+                    se[r] = start[row + b0 + 1];
+                }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) walk(sb[k], se[k]);
+            }
+            if (__any(nr == 2)) {
+                uint32_t sb[9], se[9];
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    const int row = row_of(r);
+                    sb[r] = nr == 2 ? start[row + a1] : 0u;
+                    se[r] = nr == 2 ? start[row + b1 + 1] : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) walk(sb[k], se[k]);
             }
             // Phase 2
             const int nl = nhit < RD_HITS ? nhit : RD_HITS;

@@ -45,17 +45,14 @@ __device__ __forceinline__ int wave_id() { return xcd_block(blockIdx.x, gridDim.
 // BLOCK threads per workgroup and PPT particles per lane (the autotuner's
 // choices; a wave takes PPT consecutive 64-slot work items one after another);
 // the type-pair table is staged in LDS once per workgroup.
+#define RD_FORCE_ARGS                                                                                   \
+    const float4 *__restrict__ ps, float4 *__restrict__ pn, uint2 *__restrict__ kr,                     \
+        uint32_t *__restrict__ count, uint32_t *__restrict__ tsum, const uint32_t *__restrict__ start, \
+        const RdDev *__restrict__ P, const RdRep *__restrict__ reps, const RdWave *__restrict__ waves, \
+        int nwaves, const RdPot1Dev *__restrict__ pot1, RdStepState *__restrict__ st,                   \
+        const float *__restrict__ noise, int ntot
 template <int BLOCK, int PPT>
-__global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict__ ps, float4* __restrict__ pn,
-                                                           uint2* __restrict__ kr,
-                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ tsum,
-                                                           const uint32_t* __restrict__ start,
-                                                           const RdDev* __restrict__ P,
-                                                           const RdRep* __restrict__ reps,
-                                                           const RdWave* __restrict__ waves, int nwaves,
-                                                           const RdPot1Dev* __restrict__ pot1,
-                                                           RdStepState* __restrict__ st,
-                                                           const float* __restrict__ noise, int ntot)
+__device__ __forceinline__ void force_update(RD_FORCE_ARGS)
 {
     __shared__ RdTab T;
     __shared__ RdHitList il[BLOCK / 64];
@@ -81,6 +78,22 @@ __global__ __launch_bounds__(BLOCK) void k_force_update(const float4* __restrict
             }
         }
     }
+}
+
+// One particle per lane: the register budget is held to 64 VGPRs (8 waves per
+// SIMD where the LDS allows it): the gather is bound by the latency of its
+// dependent loads, and the extra wave per SIMD is worth the few bytes spilled.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_force_update(RD_FORCE_ARGS)
+{
+    force_update<BLOCK, 1>(ps, pn, kr, count, tsum, start, P, reps, waves, nwaves, pot1, st, noise, ntot);
+}
+
+// PPT > 1 particles per lane (the compiler's own register budget)
+template <int BLOCK, int PPT>
+__global__ __launch_bounds__(BLOCK) void k_force_update_ppt(RD_FORCE_ARGS)
+{
+    force_update<BLOCK, PPT>(ps, pn, kr, count, tsum, start, P, reps, waves, nwaves, pot1, st, noise, ntot);
 }
 
 // gradient + energy (FP64 accumulation) at the current positions
@@ -1043,10 +1056,15 @@ static int wave_blocks(const rd_system* h) { return std::max(1, nblk(h->nwaves, 
 template <int B, int PPT>
 static void launch_force(rd_system* h)
 {
-    hipLaunchKernelGGL((k_force_update<B, PPT>), dim3(std::max(1, nblk(h->nwaves, B / 64 * PPT))), dim3(B), 0,
-                       h->stream, h->d_ps, h->d_pn, h->d_kr, h->d_count, h->d_bsum, h->d_start, h->d_P,
-                       h->d_reps,
-                       h->d_waves, h->nwaves, h->d_pot1, h->d_st, h->d_noise, h->n);
+    const dim3 grid(std::max(1, nblk(h->nwaves, B / 64 * PPT)));
+    if constexpr (PPT == 1)
+        hipLaunchKernelGGL((k_force_update<B>), grid, dim3(B), 0, h->stream, h->d_ps, h->d_pn, h->d_kr, h->d_count,
+                           h->d_bsum, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st,
+                           h->d_noise, h->n);
+    else
+        hipLaunchKernelGGL((k_force_update_ppt<B, PPT>), grid, dim3(B), 0, h->stream, h->d_ps, h->d_pn, h->d_kr,
+                           h->d_count, h->d_bsum, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1,
+                           h->d_st, h->d_noise, h->n);
 }
 
 template <int B>

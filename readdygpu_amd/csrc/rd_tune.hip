@@ -62,6 +62,12 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
         if (user_lc.block == 0 || user_lc.block == b) cfgs.push_back({ RD_KERNEL_GATHER, b, ppt0 });
     double best = 1e30, best_cell = cells[0];
     Cfg best_cfg = cfgs[0];
+    // the default shape (256 threads, one particle per lane, or what the
+    // caller fixed) is kept unless another shape is more than 2% faster: the
+    // trials are short and taken on the current state, and a few percent
+    // between shapes is within what the state's evolution over a run moves
+    const int def_block = user_lc.block ? user_lc.block : 256;
+    double best_def = 1e30, best_def_cell = cells[0];
     auto trial = [&](double c, const Cfg& k) -> int {
         h->prm.cell_size = c;
         rd_launch_config lc{};
@@ -81,6 +87,7 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
         if (r) return r;
         ms = std::min(ms, ms2);
         if (ms < best) { best = ms; best_cell = c; best_cfg = k; }
+        if (k.block == def_block && k.ppt == ppt0 && ms < best_def) { best_def = ms; best_def_cell = c; }
         return RD_OK;
     };
     for (double c : cells)
@@ -91,6 +98,11 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
         const double c = best_cell;
         for (int q : { 2, 4 })
             if ((rc = trial(c, Cfg{ first.kernel, first.block, q })) != RD_OK) return rc;
+    }
+    if (best_def < 1e30 && !(best < 0.98 * best_def)) {
+        best = best_def;
+        best_cell = best_def_cell;
+        best_cfg = Cfg{ RD_KERNEL_GATHER, def_block, ppt0 };
     }
     // keep the winner, restore the state
     h->prm.cell_size = user_cell > 0 ? user_cell : best_cell;
