@@ -345,10 +345,11 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
             };
             // 27-cell stencil as 9 rows of at most two runs of contiguous
             // slots (two only for homes at the x edge of the lattice).  The
-            // 18 bounds of the first runs are loaded first (independent
-            // loads, in flight together), then those runs are walked; the
-            // second runs follow only in waves that have any, so that their
-            // bounds do not hold registers in the common case.
+            // first runs go in groups of three rows: the group's 6 bounds are
+            // loaded together (independent loads, in flight at once), then
+            // its runs are walked.  The second runs follow only in waves that
+            // have any.  Few bounds live at a time keep the kernel at 57
+            // VGPRs, 8 waves per SIMD, without spills.
             int a0, b0, a1, b1;
             const int nr = x_runs(cx, 1, Lx, &a0, &b0, &a1, &b1);
             auto row_of = [&](int r) {
@@ -365,16 +366,17 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                 }
                 if (j < j1) test(j, ps[j]);
             };
-            {
-                uint32_t sb[9], se[9];
 #pragma unroll
-                for (int r = 0; r < 9; ++r) {
-                    const int row = row_of(r);
+            for (int g = 0; g < 9; g += 3) {
+                uint32_t sb[3], se[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const int row = row_of(g + r);
                     sb[r] = start[row + a0];
                     se[r] = start[row + b0 + 1];
                 }
 #pragma unroll
-                for (int k = 0; k < 9; ++k) walk(sb[k], se[k]);
+                for (int k = 0; k < 3; ++k) walk(sb[k], se[k]);
             }
             if (__any(nr == 2)) {
                 uint32_t sb[9], se[9];

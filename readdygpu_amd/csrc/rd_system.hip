@@ -82,7 +82,7 @@ __device__ __forceinline__ void force_update(RD_FORCE_ARGS)
 
 // One particle per lane: the register budget is held to 64 VGPRs (8 waves per
 // SIMD where the LDS allows it): the gather is bound by the latency of its
-// dependent loads, and the extra wave per SIMD is worth the few bytes spilled.
+// dependent loads, and waves in flight hide it.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_force_update(RD_FORCE_ARGS)
 {
