@@ -825,11 +825,13 @@ static int build_tables(rd_system* h)
             R.hi[d] = (float)A.hi[d];
             R.Lb[d] = R.hi[d] - R.lo[d];
             R.half[d] = R.Lb[d] / 2.0f;
-            int l = (int)(R.Lb[d] / R.cell);
+            const float q = R.Lb[d] / R.cell;
+            if (!(q < 268435456.0f)) return set_err(h, RD_ERANGE, "too many cells; raise cell_size");
+            int l = (int)q;
             R.L[d] = l < 1 ? 1 : l;
             nc *= R.L[d];
+            if (coff + nc > (1L << 28)) return set_err(h, RD_ERANGE, "too many cells; raise cell_size");
         }
-        if (coff + nc > (1L << 28)) return set_err(h, RD_ERANGE, "too many cells; raise cell_size");
         R.coff = (int)coff;
         R.off = off;
         R.n = A.n;

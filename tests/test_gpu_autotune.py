@@ -6,7 +6,7 @@ import json
 import numpy as np
 import pytest
 
-from readdygpu_amd import Simulation, workloads
+from readdygpu_amd import RdError, Simulation, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -64,3 +64,18 @@ def test_tuning_cache_hit_applies_the_stored_choice(tmp_path):
         assert c.tune_key() != key
         c.autotune(trial_steps=2, cache=path)
     assert len(json.load(open(path))) == 2
+
+
+def test_cell_size_too_fine_is_refused_and_the_lattice_kept():
+    """A cell edge whose lattice would exceed the cell-index range is refused
+    (RD_ERANGE); the previous lattice stays and the run continues bit for bit."""
+    spec, xyz, types = workloads.plain_10k(n=3000)
+    ref, _ = run(spec, xyz, types, (4, 6))
+
+    def change(sim):
+        before = sim.info().cell_size
+        with pytest.raises(RdError):
+            sim.set_cell_size(1e-4)
+        assert sim.info().cell_size == before and sim.info().step == 4
+    x, _ = run(spec, xyz, types, (4, 6), change=change)
+    assert np.array_equal(x, ref)
