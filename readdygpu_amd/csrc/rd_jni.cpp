@@ -84,7 +84,7 @@ extern "C" void Java_readdy_impl_sim_top_TopGPU_cCreateSimulation(JNIEnv* env, j
     rd_model_info info;
     rd_model_get_info(model, &info);
     if (info.n_groups > 0)
-        say("warning: %d particle groups found; bonded group forces are not simulated\n", info.n_groups);
+        say("%d particle groups: %d bonded pairs\n", info.n_groups, info.n_bonds);
 
     // lib/ReaDDyGPU.cpp:101: maxCutoff = 10; the reference seeds its RNG with
     // the wall clock (CudaSimulation.cu:773), RDGPU_SEED fixes the seed.

@@ -23,8 +23,16 @@ struct RdStepState {
     unsigned long long step;        // global step counter (noise counter)
     unsigned long long noise_step0; // first step held by the noise array
     int noise_nsteps;               // 0: generator
-    unsigned int err;               // bit 0: non-finite position, bit 1: noise array exhausted
+    unsigned int err;               // bit 0: non-finite position, bit 1: noise array exhausted,
+                                    // bit 2: position still outside the box after the periodic
+                                    // correction (moved more than one box length in a step),
+                                    // bit 3: a pair or bond gradient term beyond the fixed-point range
 };
+
+#define RD_ERR_NONFINITE 1u
+#define RD_ERR_NOISE 2u
+#define RD_ERR_OUTSIDE 4u
+#define RD_ERR_FIXRANGE 8u
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of slots so that neighbour cells share one L2.
@@ -129,15 +137,32 @@ __device__ __forceinline__ void for_stencil(const RdRep& R, const uint32_t* __re
 // is therefore bitwise reproducible.
 #define RD_FIX_SCALE 4294967296.0f
 #define RD_FIX_INV 2.3283064365386963e-10
+#define RD_FIX_MAX 5.0e8f        // |term| limit; a larger term is clamped and reported
 __device__ __forceinline__ long long to_fix(float v)
 {
     // floor(v * 2^32) from the integer part and the exact fraction: two
     // 32-bit conversions instead of a float -> int64 conversion sequence
-    v = fminf(fmaxf(v, -1.0e9f), 1.0e9f);
+    v = fminf(fmaxf(v, -RD_FIX_MAX), RD_FIX_MAX);
     const float fl = floorf(v);
     const int hi = (int)fl;
     const uint32_t lo = (uint32_t)((v - fl) * RD_FIX_SCALE);
     return (long long)(((unsigned long long)(uint32_t)hi << 32) | lo);
+}
+
+// Per-particle accumulator: fixed-point gradient sums, FP64 energy, and
+// whether any term was clamped to the fixed-point range (reported by the
+// step as an error instead of being summed silently).
+struct RdAcc {
+    long long x, y, z;
+    double e;
+    bool sat;
+};
+
+__device__ __forceinline__ void acc_add(RdAcc& a, float pre, float dx, float dy, float dz)
+{
+    const float fx = pre * dx, fy = pre * dy, fz = pre * dz;
+    a.sat = a.sat || !(fabsf(fx) < RD_FIX_MAX && fabsf(fy) < RD_FIX_MAX && fabsf(fz) < RD_FIX_MAX);
+    a.x += to_fix(fx); a.y += to_fix(fy); a.z += to_fix(fz);
 }
 
 // Per-block LDS copy of the type-pair tables and EM coefficients: the inner
@@ -175,7 +200,7 @@ __device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
 // this one function, so all of them produce the same bits.
 template <bool ENERGY>
 __device__ __forceinline__ void pair_add(const RdTab& T, int rowbase, float dx, float dy, float dz, float d2, int tj,
-                                         long long& ax, long long& ay, long long& az, double& e)
+                                         RdAcc& acc)
 {
     if (d2 < T.rc2[rowbase + tj] && d2 > 0.0f) {
         float rinv = __builtin_amdgcn_rsqf(d2);
@@ -188,8 +213,8 @@ __device__ __forceinline__ void pair_add(const RdTab& T, int rowbase, float dx, 
             m &= m - 1;
             pre += pair_pre_fast<ENERGY>(T.pot2[pi], r0, r, rinv, &ep);
         }
-        ax += to_fix(pre * dx); ay += to_fix(pre * dy); az += to_fix(pre * dz);
-        if (ENERGY) e += (double)(0.5f * ep);
+        acc_add(acc, pre, dx, dy, dz);
+        if (ENERGY) acc.e += (double)(0.5f * ep);
     }
 }
 
@@ -201,35 +226,12 @@ __device__ __forceinline__ float dist2(float dx, float dy, float dz)
 // the pair (p, q) with the minimum-image separation
 template <bool ENERGY>
 __device__ __forceinline__ void pair_accum(const RdTab& T, int rowbase, float4 p, float4 q, const RdRep& R,
-                                           long long& ax, long long& ay, long long& az, double& e)
+                                           RdAcc& acc)
 {
     float dx = min_image(p.x - q.x, R.Lb[0], R.half[0]);
     float dy = min_image(p.y - q.y, R.Lb[1], R.half[1]);
     float dz = min_image(p.z - q.z, R.Lb[2], R.half[2]);
-    pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), ax, ay, az, e);
-}
-
-// The pair (p, q) for a home particle p in an interior cell (every cell
-// coordinate in [1, L-2]) and q in its 27-cell stencil: the plain difference
-// is the minimum image for every pair within range (cell >= range), so no
-// per-pair image test is needed.  A candidate beyond the home type's largest
-// range is rejected before the type-pair table is read.
-template <bool ENERGY>
-__device__ __forceinline__ void pair_inner(const RdTab& T, int rowbase, float rmax2, float4 p, float4 q,
-                                           long long& ax, long long& ay, long long& az, double& e)
-{
-    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    const float d2 = dist2(dx, dy, dz);
-    if (d2 < rmax2) pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), ax, ay, az, e);
-}
-
-// one stencil candidate: raw difference for interior homes, minimum image otherwise
-template <bool ENERGY>
-__device__ __forceinline__ void pair_any(const RdTab& T, int rowbase, float rmax2, bool interior, float4 p, float4 q,
-                                         const RdRep& R, long long& ax, long long& ay, long long& az, double& e)
-{
-    if (interior) pair_inner<ENERGY>(T, rowbase, rmax2, p, q, ax, ay, az, e);
-    else pair_accum<ENERGY>(T, rowbase, p, q, R, ax, ay, az, e);
+    pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), acc);
 }
 
 __device__ __forceinline__ bool interior_cell(const RdRep& R, int cx, int cy, int cz)
@@ -241,12 +243,12 @@ __device__ __forceinline__ bool interior_cell(const RdRep& R, int cx, int cy, in
 // by a separate external pass (composite geometry, surrogate) is added first
 template <bool ENERGY>
 __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const RdTab& T,
-                                              const RdPot1Dev* __restrict__ pot1, uint32_t s, float4 p, long long ax,
-                                              long long ay, long long az, double* e)
+                                              const RdPot1Dev* __restrict__ pot1, uint32_t s, float4 p,
+                                              const RdAcc& acc, double* e)
 {
-    float gx = (float)((double)ax * RD_FIX_INV);
-    float gy = (float)((double)ay * RD_FIX_INV);
-    float gz = (float)((double)az * RD_FIX_INV);
+    float gx = (float)((double)acc.x * RD_FIX_INV);
+    float gy = (float)((double)acc.y * RD_FIX_INV);
+    float gz = (float)((double)acc.z * RD_FIX_INV);
     const float4* __restrict__ gext = P->gext;
     const int ti = w_type(p.w);
     if (gext && ((P->ext_mask >> ti) & 1u)) {
@@ -271,8 +273,7 @@ __device__ __forceinline__ float3 finish_grad(const RdDev* __restrict__ P, const
 // the fixed-point pair sum
 template <bool ENERGY>
 __device__ __forceinline__ void bond_accum(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
-                                           const float4* __restrict__ ps, float4 p, long long& ax, long long& ay,
-                                           long long& az, double& e)
+                                           const float4* __restrict__ ps, float4 p, RdAcc& acc)
 {
     const uint32_t gi = (uint32_t)R.off + w_index(p.w);
     const int b1 = P->bstart[gi + 1];
@@ -291,8 +292,8 @@ __device__ __forceinline__ void bond_accum(const RdDev* __restrict__ P, const Rd
         const float r0 = fr0 >= 0.0f ? fr0 : T.r0[rowbase + w_type(q.w)];
         float ep = 0.0f;
         const float pre = pair_pre_fast<ENERGY>(P->bpot[bd.y], r0, r, rinv, &ep);
-        ax += to_fix(pre * dx); ay += to_fix(pre * dy); az += to_fix(pre * dz);
-        if (ENERGY) e += (double)(0.5f * ep);
+        acc_add(acc, pre, dx, dy, dz);
+        if (ENERGY) acc.e += (double)(0.5f * ep);
     }
 }
 
@@ -309,12 +310,13 @@ template <bool ENERGY>
 __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
                                               const RdPot1Dev* __restrict__ pot1,
                                               const float4* __restrict__ ps, const uint32_t* __restrict__ start,
-                                              uint32_t s, float4 p, double* energy, RdHitList* __restrict__ il)
+                                              uint32_t s, float4 p, double* energy, bool* sat,
+                                              RdHitList* __restrict__ il)
 {
     const int ti = w_type(p.w);
     const int rowbase = ti * P->nt;
-    long long ax = 0, ay = 0, az = 0;
-    double e = 0.0;        // pair, bond and external energies accumulate in FP64
+    RdAcc acc;             // pair, bond and external energies accumulate in FP64
+    acc.x = 0; acc.y = 0; acc.z = 0; acc.e = 0.0; acc.sat = false;
     const int Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
     int cx, cy, cz;
     cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
@@ -339,7 +341,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                 const float d2 = dist2(dx, dy, dz);
                 if (d2 < T.rc2[rowbase + w_type(q.w)] && d2 > 0.0f) {
                     if (nhit < RD_HITS) il->j[nhit][lane] = j;
-                    else pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), ax, ay, az, e);
+                    else pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), acc);
                     ++nhit;
                 }
             };
@@ -400,18 +402,20 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                         dy = min_image(dy, R.Lb[1], R.half[1]);
                         dz = min_image(dz, R.Lb[2], R.half[2]);
                     }
-                    pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), ax, ay, az, e);
+                    pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), acc);
                 }
             }
         } else {
             for_stencil(R, start, cx, cy, cz, P->S, [&](uint32_t j) {
-                if (j != s) pair_accum<ENERGY>(T, rowbase, p, ps[j], R, ax, ay, az, e);
+                if (j != s) pair_accum<ENERGY>(T, rowbase, p, ps[j], R, acc);
             });
         }
     }
-    if (P->bstart) bond_accum<ENERGY>(P, R, T, ps, p, ax, ay, az, e);
-    float3 g = finish_grad<ENERGY>(P, T, pot1, s, p, ax, ay, az, &e);
+    if (P->bstart) bond_accum<ENERGY>(P, R, T, ps, p, acc);
+    double e = acc.e;
+    float3 g = finish_grad<ENERGY>(P, T, pot1, s, p, acc, &e);
     if (ENERGY) *energy = e;
+    *sat = acc.sat;
     return g;
 }
 
@@ -424,11 +428,16 @@ __device__ __forceinline__ float wrap_coord(float x, float lo, float hi, float L
 
 // Euler-Maruyama for slot s (CudaSimulation.cu:1850-1858): noise, x += -a*dU
 // + b*xi, one periodic correction, the next cell key and its histogram entry.
-// The noise counter is (local index, step, replica stream); a caller-supplied
-// noise array is indexed by the global particle index.
+// The reference repeats the correction until the particle is inside
+// (:1857-1858); a particle that is still outside after one correction moved
+// more than a box length in one step, which is reported as an error
+// (RD_ERR_OUTSIDE) rather than wrapped.  The noise counter is (local index,
+// step, replica stream); a caller-supplied noise array is indexed by the
+// global particle index.
 __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, const RdRep& R, const RdTab& T,
                                                 RdStepState* __restrict__ st, const float* __restrict__ noise,
-                                                int ntot, uint32_t s, float4 p, float3 g, float4* __restrict__ pn,
+                                                int ntot, uint32_t s, float4 p, float3 g, bool sat,
+                                                float4* __restrict__ pn,
                                                 uint2* __restrict__ kr,
                                                 uint32_t* __restrict__ count, uint32_t* __restrict__ tsum)
 {
@@ -445,7 +454,7 @@ __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, con
             z = make_float3(noise[base], noise[base + ntot], noise[base + 2 * (size_t)ntot]);
         } else {
             z = make_float3(0.0f, 0.0f, 0.0f);
-            atomicOr(&st->err, 2u);
+            atomicOr(&st->err, RD_ERR_NOISE);
         }
     }
     const float a = T.a[t], bb = T.b[t];
@@ -455,7 +464,12 @@ __device__ __forceinline__ void em_update_store(const RdDev* __restrict__ P, con
     x = wrap_coord(x, R.lo[0], R.hi[0], R.Lb[0]);
     y = wrap_coord(y, R.lo[1], R.hi[1], R.Lb[1]);
     zc = wrap_coord(zc, R.lo[2], R.hi[2], R.Lb[2]);
-    if (!(fabsf(x) < 3.0e38f && fabsf(y) < 3.0e38f && fabsf(zc) < 3.0e38f)) atomicOr(&st->err, 1u);
+    const bool inside = x >= R.lo[0] && x <= R.hi[0] && y >= R.lo[1] && y <= R.hi[1] && zc >= R.lo[2] &&
+                        zc <= R.hi[2];
+    if (!inside || sat) {
+        const bool fin = fabsf(x) < 3.0e38f && fabsf(y) < 3.0e38f && fabsf(zc) < 3.0e38f;
+        atomicOr(&st->err, (fin ? (inside ? 0u : RD_ERR_OUTSIDE) : RD_ERR_NONFINITE) | (sat ? RD_ERR_FIXRANGE : 0u));
+    }
     pn[s] = make_float4(x, y, zc, p.w);
     if (P->gid_n) P->gid_n[s] = (uint32_t)R.off + idx;
     uint32_t k = cell_key(R, x, y, zc);

@@ -64,8 +64,9 @@ __device__ __forceinline__ void force_update(RD_FORCE_ARGS)
         if (s >= (uint32_t)it.s1) return;
         const RdRep R = reps[it.rep];
         float4 p = ps[s];
-        float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
-        em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, kr, count, tsum);
+        bool sat;
+        float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &sat, &il[threadIdx.x >> 6]);
+        em_update_store(P, R, T, st, noise, ntot, s, p, g, sat, pn, kr, count, tsum);
     } else {
         for (int k = 0; k < PPT; ++k) {
             const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w + k));
@@ -73,8 +74,9 @@ __device__ __forceinline__ void force_update(RD_FORCE_ARGS)
             if (s < (uint32_t)it.s1) {
                 const RdRep R = reps[it.rep];
                 float4 p = ps[s];
-                float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &il[threadIdx.x >> 6]);
-                em_update_store(P, R, T, st, noise, ntot, s, p, g, pn, kr, count, tsum);
+                bool sat;
+                float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &sat, &il[threadIdx.x >> 6]);
+                em_update_store(P, R, T, st, noise, ntot, s, p, g, sat, pn, kr, count, tsum);
             }
         }
     }
@@ -102,7 +104,8 @@ __global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict
                                                         const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
                                                         const RdWave* __restrict__ waves, int nwaves,
                                                         const RdPot1Dev* __restrict__ pot1,
-                                                        float* __restrict__ grad, double* __restrict__ etot)
+                                                        float* __restrict__ grad, double* __restrict__ etot,
+                                                        RdStepState* __restrict__ st)
 {
     __shared__ double red[RD_BLOCK / 64];
     __shared__ RdTab T;
@@ -115,7 +118,9 @@ __global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict
         const RdRep R = reps[it.rep];
         float4 p = ps[s];
         double ef = 0.0;
-        float3 g = gather_grad<true>(P, R, T, pot1, ps, start, s, p, &ef, &il[threadIdx.x >> 6]);
+        bool sat;
+        float3 g = gather_grad<true>(P, R, T, pot1, ps, start, s, p, &ef, &sat, &il[threadIdx.x >> 6]);
+        if (sat) atomicOr(&st->err, RD_ERR_FIXRANGE);
         size_t gi = (size_t)R.off + w_index(p.w);
         grad[3 * gi] = g.x; grad[3 * gi + 1] = g.y; grad[3 * gi + 2] = g.z;
         e = ef;
@@ -579,6 +584,19 @@ static int set_err(rd_system* h, int code, const std::string& m)
             return set_err((h), RD_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
     } while (0)
 
+// error bits the step kernels raise (RdStepState::err) as an RD_ERANGE message
+static int step_error(rd_system* h, unsigned int err)
+{
+    if (err & RD_ERR_FIXRANGE)
+        return set_err(h, RD_ERANGE, "a pair or bond gradient term exceeds the fixed-point range (|term| >= 5e8)");
+    if (err & RD_ERR_NONFINITE) return set_err(h, RD_ERANGE, "non-finite particle position");
+    if (err & RD_ERR_OUTSIDE)
+        return set_err(h, RD_ERANGE, "a particle moved more than one box length in one step (outside the box after "
+                                     "the periodic correction)");
+    if (err & RD_ERR_NOISE) return set_err(h, RD_ERANGE, "noise array exhausted");
+    return RD_OK;
+}
+
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
 static void drop_graph(rd_system* h)
@@ -785,7 +803,9 @@ double rd_host_r0(const rd_system* h, int i, int j)
 // (CudaSimulation.cu:524-528), cells of all replicas concatenated.
 static int build_tables(rd_system* h)
 {
-    RdDev& P = h->P;
+    // everything is computed into locals and stored only once every replica
+    // passes the lattice checks, so a refused size leaves the handle as it was
+    RdDev P;
     memset(&P, 0, sizeof(P));
     const rd_params& g = h->prm;
     int nt = h->nt;
@@ -805,18 +825,15 @@ static int build_tables(rd_system* h)
                 if (h->pot2mask[p][i * nt + j]) m |= 1u << p;
             P.p2mask[i * nt + j] = m;
         }
-    h->max_range = maxr;
-    h->max_rxn = maxrx;
     double mr = std::max(maxr, maxrx);
     double cell = g.cell_size > 0 ? g.cell_size : (mr > 0 ? 1.01 * mr : g.max_cutoff);
-    h->cell = cell;
     P.S = std::max(1, (int)std::ceil(maxr * 1.0001 / cell));
     long coff = 0;
     int off = 0;
-    h->rep_dev.resize(h->reps.size());
+    std::vector<RdRep> rep_dev(h->reps.size());
     for (size_t r = 0; r < h->reps.size(); ++r) {
         const rd_system::Rep& A = h->reps[r];
-        RdRep& R = h->rep_dev[r];
+        RdRep& R = rep_dev[r];
         memset(&R, 0, sizeof(R));
         R.cell = (float)cell;
         long nc = 1;
@@ -839,7 +856,6 @@ static int build_tables(rd_system* h)
         coff += nc;
         off += A.n;
     }
-    h->ncell = (int)std::max(coff, 1L);
     P.n_pot2 = (int)h->pot2.size();
     for (int p = 0; p < P.n_pot2; ++p) {
         const rd_pot2& q = h->pot2[p];
@@ -855,10 +871,10 @@ static int build_tables(rd_system* h)
     }
     P.seed = g.seed;
     P.n_pot1 = (int)h->pot1.size();
-    h->pot1dev.resize(h->pot1.size());
+    std::vector<RdPot1Dev> pot1dev(h->pot1.size());
     for (size_t k = 0; k < h->pot1.size(); ++k) {
         const rd_pot1& q = h->pot1[k];
-        RdPot1Dev& d = h->pot1dev[k];
+        RdPot1Dev& d = pot1dev[k];
         memset(&d, 0, sizeof(d));
         d.kind = q.type; d.subtype = q.subtype; d.k = q.force_const;
         double nn = std::sqrt((double)q.normal[0] * q.normal[0] + (double)q.normal[1] * q.normal[1] +
@@ -873,15 +889,23 @@ static int build_tables(rd_system* h)
         d.mask = q.type_mask;
     }
     // waves: 64 consecutive slots of one replica each
-    h->wave_tab.clear();
-    for (size_t r = 0; r < h->rep_dev.size(); ++r) {
-        const RdRep& R = h->rep_dev[r];
+    std::vector<RdWave> wave_tab;
+    for (size_t r = 0; r < rep_dev.size(); ++r) {
+        const RdRep& R = rep_dev[r];
         for (int s = 0; s < R.n; s += 64) {
             RdWave w;
             w.rep = (int)r; w.s0 = R.off + s; w.s1 = R.off + std::min(R.n, s + 64); w.pad = 0;
-            h->wave_tab.push_back(w);
+            wave_tab.push_back(w);
         }
     }
+    h->P = P;
+    h->max_range = maxr;
+    h->max_rxn = maxrx;
+    h->cell = cell;
+    h->rep_dev.swap(rep_dev);
+    h->ncell = (int)std::max(coff, 1L);
+    h->pot1dev.swap(pot1dev);
+    h->wave_tab.swap(wave_tab);
     h->nwaves = (int)h->wave_tab.size();
     return RD_OK;
 }
@@ -1278,9 +1302,7 @@ extern "C" int rd_simulate(rd_system* h, int nsteps)
     RdStepState st;
     HIPCK(h, hipMemcpyAsync(&st, h->d_st, sizeof(st), hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
-    if (st.err & 1u) return set_err(h, RD_ERANGE, "non-finite particle position");
-    if (st.err & 2u) return set_err(h, RD_ERANGE, "noise array exhausted");
-    return RD_OK;
+    return step_error(h, st.err);
 }
 
 // pieces of the step for the measurement entry points (rd_bench.hip)
@@ -1380,16 +1402,18 @@ extern "C" int rd_gradients(rd_system* h, float* grad, double* energy)
     HIPCK(h, hipMemsetAsync(h->d_etot, 0, sizeof(double), h->stream));
     if (h->ext_active()) rd_host_launch_ext(h);
     hipLaunchKernelGGL(k_gradients, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start, h->d_P,
-                           h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_xyz_io, h->d_etot);
+                           h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_xyz_io, h->d_etot, h->d_st);
     HIPCK(h, hipGetLastError());
     std::vector<float> tmp(3 * (size_t)h->n);
     double e = 0.0;
+    RdStepState st;
     HIPCK(h, hipMemcpyAsync(tmp.data(), h->d_xyz_io, sizeof(float) * 3 * h->n, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipMemcpyAsync(&e, h->d_etot, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipMemcpyAsync(&st, h->d_st, sizeof(st), hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
     if (grad) memcpy(grad, tmp.data(), sizeof(float) * 3 * h->n);
     if (energy) *energy = e;
-    return RD_OK;
+    return (st.err & RD_ERR_FIXRANGE) ? step_error(h, RD_ERR_FIXRANGE) : RD_OK;
 }
 
 extern "C" int rd_noise_fill(rd_system* h, uint64_t step0, int nsteps, float* out)
@@ -1833,8 +1857,8 @@ extern "C" int rd_frame_wait(rd_system* h, const float** xyz, const int** type, 
     if (!h) return RD_EINVAL;
     if (!h->snap_valid) return set_err(h, RD_ESTATE, "no frame copy in flight");
     HIPCK(h, hipEventSynchronize(h->ev_frame));
-    if (h->hp_st->err & 1u) return set_err(h, RD_ERANGE, "non-finite particle position");
-    if (h->hp_st->err & 2u) return set_err(h, RD_ERANGE, "noise array exhausted");
+    const int rc = step_error(h, h->hp_st->err);
+    if (rc) return rc;
     if (xyz) *xyz = h->hp_xyz;
     if (type) *type = h->hp_type;
     if (n) *n = h->snap_n;
@@ -1946,14 +1970,19 @@ extern "C" int rd_checkpoint_read(rd_system* h, const char* path)
     std::vector<double> bounds;
     std::vector<uint32_t> streams;
     std::vector<int> counts;
+    uint64_t csum = 0;
     for (uint32_t r = 0; ok && r < nrep; ++r) {
         double lo[3], hi[3];
         uint32_t st = 0, cnt = 0;
         ok = fread(lo, 8, 3, f) == 3 && fread(hi, 8, 3, f) == 3 && fread(&st, 4, 1, f) == 1 && fread(&cnt, 4, 1, f) == 1;
         for (int d = 0; d < 3; ++d) { bounds.push_back(lo[d]); bounds.push_back(hi[d]); }
         streams.push_back(st);
-        counts.push_back((int)cnt);
+        // the replica sizes must add up to the particle count the arrays hold
+        ok = ok && cnt <= (uint32_t)RD_MAX_PARTICLES;
+        counts.push_back(ok ? (int)cnt : 0);
+        csum += cnt;
     }
+    ok = ok && csum == (uint64_t)n;
     std::vector<float> xyz(ok ? 3 * (size_t)n : 0);
     std::vector<int> type(ok ? (size_t)n : 0);
     ok = ok && fread(xyz.data(), 4, xyz.size(), f) == xyz.size() && fread(type.data(), 4, type.size(), f) == type.size();

@@ -81,3 +81,30 @@ def test_non_finite_positions_are_reported():
     with Simulation(spec, xyz, types) as sim:
         with pytest.raises(RdError, match="non-finite"):
             sim.simulate(1)
+
+
+def test_snapshot_whose_replica_sizes_do_not_add_up_is_refused(tmp_path):
+    """The replica sizes stored in a snapshot must add up to its particle
+    count: a crafted file whose sizes overrun the stored arrays (or wrap past
+    2^31) is refused before any particle is read from it."""
+    import struct
+    spec, xyz, types = workloads.plain_10k(n=400)
+    p = tmp_path / "a.ckpt"
+    with Simulation(spec, xyz, types) as a:
+        a.simulate(1)
+        a.checkpoint(str(p))
+    raw = bytearray(p.read_bytes())
+    # magic 8, version 4, seed 8, step 8, nt 4, nrep 4, n 4; replica 0: lo, hi (6 doubles), stream 4, count 4
+    off = 8 + 4 + 8 + 8 + 4 + 4 + 4 + 48 + 4
+    assert struct.unpack_from("<I", raw, off)[0] == 400
+    with Simulation(spec, xyz, types) as b:
+        for bad in (401, 1 << 31, 0xFFFFFFFF):
+            q = tmp_path / f"bad{bad}.ckpt"
+            r = bytearray(raw)
+            struct.pack_into("<I", r, off, bad)
+            q.write_bytes(bytes(r))
+            with pytest.raises(RdError, match="not a snapshot"):
+                b.restore(str(q))
+        # the handle is unchanged and still runs
+        b.simulate(2)
+        assert b.info().n == 400

@@ -1,5 +1,5 @@
 """The pair cases of the reference's ParticleConfigurationTest
-(test/readdy/impl/sim/core/particle/ParticleConfigurationTest.java:236-270
+(test/readdy/impl/sim/core/config/ParticleConfigurationTest.java:236-270
 and :316-375), on the device.
 
 Three type-0 particles of radius 1.4 nm at the positions of
