@@ -180,6 +180,16 @@ int rd_rdf_reset(rd_system* h);
 int rd_neighbor_pairs(rd_system* h, int64_t* out, long cap, long* count);
 /* pairs with d^2 < R^2(ti,tj), R from rd_set_reaction_radii */
 int rd_reaction_candidates(rd_system* h, int64_t* out, long cap, long* count);
+/* The same candidate set as rd_reaction_candidates, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): no counterpart in the reference.  Its
+ * scratch (per-particle offsets, scan sums, the pair list) is a fixed device
+ * array of the library, about 48 MB per GPU, held until the process ends; the
+ * call makes no device allocation.  Limits: at most 2^23 particles and 2^20
+ * candidate pairs (RD_ERANGE beyond them); the handle must be up to date
+ * (after rd_initialize or a step; RD_ESTATE otherwise).  Calls that overlap on
+ * one GPU share the scratch and race. */
+int rd_reaction_candidates_noalloc(rd_system* h, int64_t* out, long cap, long* count);
 
 /* ---- reactions: the cReactions protocol of lib/ReaDDyGPU.cpp:303-463:
  *      creactions = [count, (readdyId, newType, x, y, z, index) x count];

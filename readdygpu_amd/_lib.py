@@ -150,6 +150,7 @@ _SIGS = {
     "rd_rdf_reset": (C.c_int, [P]),
     "rd_neighbor_pairs": (C.c_int, [P, P, C.c_long, C.POINTER(C.c_long)]),
     "rd_reaction_candidates": (C.c_int, [P, P, C.c_long, C.POINTER(C.c_long)]),
+    "rd_reaction_candidates_noalloc": (C.c_int, [P, P, C.c_long, C.POINTER(C.c_long)]),
     "rd_apply_reactions": (C.c_int, [P, P]),
     "rd_get_info": (C.c_int, [P, C.POINTER(rd_info)]),
     "rd_set_launch_config": (C.c_int, [P, C.POINTER(rd_launch_config)]),

@@ -7,10 +7,12 @@
 #include "rd_kernels.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -253,6 +255,47 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ 
         for (int k = 0; k < RD_SCAN_ITEMS; ++k)
             if (base + k < (size_t)ncell) { start[base + k] = run; run += c[k]; count[base + k] = 0u; }
     }
+}
+
+// In-place exclusive scan of per-particle pair counts v[0..n) into offsets,
+// v[n] = the total; bsum holds the per-tile sums written by k_scan_reduce.
+// Each thread reads its own items before the block's barriers and writes
+// only them, so counts and offsets can share one array.
+__global__ __launch_bounds__(RD_BLOCK) void k_scan_offsets(uint32_t* v, int n, const uint32_t* __restrict__ bsum)
+{
+    __shared__ uint32_t lds[RD_BLOCK / 64];
+    __shared__ uint32_t pre_sh;
+    const size_t base = (size_t)blockIdx.x * RD_SCAN_TILE + (size_t)threadIdx.x * RD_SCAN_ITEMS;
+    uint32_t c[RD_SCAN_ITEMS];
+#pragma unroll
+    for (int k = 0; k < RD_SCAN_ITEMS; ++k) c[k] = (base + k < (size_t)n) ? v[base + k] : 0u;
+    {
+        uint32_t acc = 0;
+        for (int i = threadIdx.x; i < (int)blockIdx.x; i += RD_BLOCK) acc += bsum[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < RD_BLOCK / 64; ++w) t += lds[w];
+            pre_sh = t;
+        }
+        __syncthreads();
+    }
+    const uint32_t pre = pre_sh;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < RD_SCAN_ITEMS; ++k) sum += c[k];
+    uint32_t ex;
+    block_excl_scan(sum, &ex, lds);
+    uint32_t run = pre + ex;
+#pragma unroll
+    for (int k = 0; k < RD_SCAN_ITEMS; ++k) {
+        if (base + k < (size_t)n) v[base + k] = run;
+        run += c[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) v[n] = pre + bsum[blockIdx.x];
 }
 
 // counting-sort scatter; advances the step counter when `advance` is set
@@ -516,12 +559,20 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf_global(const float4* __restric
 // of partners j > i of every particle i, an exclusive scan turns the counts
 // into offsets, and an emitting pass writes each particle's pairs into its own
 // segment and sorts that (short) segment by j.
-template <bool EMIT>
+// pair (i, j) of the emitted list: uint2 in the library's own buffer,
+// longlong2 where the list is copied straight into the caller's int64 array
+__device__ __forceinline__ void set_pair(uint2& o, uint32_t i, uint32_t j) { o = make_uint2(i, j); }
+__device__ __forceinline__ void set_pair(longlong2& o, uint32_t i, uint32_t j)
+{
+    o = make_longlong2((long long)i, (long long)j);
+}
+
+template <bool EMIT, class PairT>
 __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ ps, const uint32_t* __restrict__ start,
                                                     const RdDev* __restrict__ P, const RdRep* __restrict__ reps,
                                                     const RdWave* __restrict__ waves, int nwaves,
                                                     const float* __restrict__ thr2, int S,
-                                                    uint32_t* __restrict__ cnt_off, uint2* __restrict__ out)
+                                                    uint32_t* __restrict__ cnt_off, PairT* __restrict__ out)
 {
     const RdWave it = wave_item(waves, nwaves, wave_id());
     const int s = it.s0 + (threadIdx.x & 63);
@@ -555,11 +606,11 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ p
                 // insertion into the sorted segment
                 const uint32_t gj = (uint32_t)R.off + jj;
                 uint32_t k = c;
-                while (k > 0 && out[o + k - 1].y > gj) {
+                while (k > 0 && (uint32_t)out[o + k - 1].y > gj) {
                     out[o + k] = out[o + k - 1];
                     --k;
                 }
-                out[o + k] = make_uint2(gi, gj);
+                set_pair(out[o + k], gi, gj);
             }
             ++c;
         }
@@ -1231,13 +1282,17 @@ static int upload(rd_system* h)
     return RD_OK;
 }
 
+static int scratch_setup(rd_system* h);
+
 extern "C" int rd_initialize(rd_system* h)
 {
     if (!h) return RD_EINVAL;
     if (h->nt == 0) return set_err(h, RD_ESTATE, "no particle types");
     if (h->reps.empty()) return set_err(h, RD_ESTATE, "no particles set");
     HIPCK(h, hipSetDevice(h->dev));
-    int rc = upload(h);
+    int rc = scratch_setup(h);
+    if (rc) return rc;
+    rc = upload(h);
     if (rc) return rc;
     h->initialized = true;
     return RD_OK;
@@ -1579,6 +1634,17 @@ extern "C" int rd_rdf_normalize(rd_system* h, double* g)
 }
 
 // ---- pair sets ----
+// per-particle partner counts, then their exclusive scan in place: off[i] is
+// the first slot of particle i's pairs, off[n] their total
+static void launch_pair_counts(rd_system* h, const float* dtable, int S, uint32_t* off, uint32_t* bsum)
+{
+    const int nsc = nblk(h->n, RD_SCAN_TILE);
+    hipLaunchKernelGGL((k_pairs<false, uint2>), dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps,
+                       h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, dtable, S, off, (uint2*)nullptr);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsc), dim3(RD_BLOCK), 0, h->stream, off, h->n, bsum);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(nsc), dim3(RD_BLOCK), 0, h->stream, off, h->n, bsum);
+}
+
 static int pairs_common(rd_system* h, bool reactions, int64_t* out, long cap, long* count)
 {
     if (!h || cap < 0 || (cap > 0 && !out)) return RD_EINVAL;
@@ -1593,18 +1659,12 @@ static int pairs_common(rd_system* h, bool reactions, int64_t* out, long cap, lo
     // per-particle counts -> offsets (the counting-sort scan over n entries)
     const int n = h->n;
     const int nsc = nblk(n, RD_SCAN_TILE);
-    uint32_t *d_cnt = nullptr, *d_off = nullptr, *d_bs = nullptr;
-    hipError_t e = hipMalloc(&d_cnt, sizeof(uint32_t) * ((size_t)n + 16));
-    if (e == hipSuccess) e = hipMalloc(&d_off, sizeof(uint32_t) * ((size_t)n + 16));
+    uint32_t *d_off = nullptr, *d_bs = nullptr;
+    hipError_t e = hipMalloc(&d_off, sizeof(uint32_t) * ((size_t)n + 16));
     if (e == hipSuccess) e = hipMalloc(&d_bs, sizeof(uint32_t) * ((size_t)nsc + 16));
     uint32_t total = 0;
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_pairs<false>, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
-                           h->d_P, h->d_reps, h->d_waves, h->nwaves, dtable, S, d_cnt, (uint2*)nullptr);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nsc), dim3(RD_BLOCK), 0, h->stream, d_cnt, n, d_bs);
-        hipLaunchKernelGGL(k_scan_apply, dim3(nsc), dim3(RD_BLOCK), 0, h->stream, d_cnt, n, d_bs, d_off);
-        e = hipGetLastError();
-    }
+    if (e == hipSuccess) launch_pair_counts(h, dtable, S, d_off, d_bs);
+    if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&total, d_off + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && total > h->pairs_cap) {
@@ -1617,17 +1677,93 @@ static int pairs_common(rd_system* h, bool reactions, int64_t* out, long cap, lo
     const long nout = std::min<long>((long)total, cap);
     std::vector<uint2> raw((size_t)nout);
     if (e == hipSuccess && total > 0) {
-        hipLaunchKernelGGL(k_pairs<true>, dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps, h->d_start,
-                           h->d_P, h->d_reps, h->d_waves, h->nwaves, dtable, S, d_off, h->d_pairs);
+        hipLaunchKernelGGL((k_pairs<true, uint2>), dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps,
+                           h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, dtable, S, d_off, h->d_pairs);
         e = hipGetLastError();
         if (e == hipSuccess && nout > 0)
             e = hipMemcpyAsync(raw.data(), h->d_pairs, sizeof(uint2) * (size_t)nout, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     }
-    for (void* p : { (void*)d_cnt, (void*)d_off, (void*)d_bs })
+    for (void* p : { (void*)d_off, (void*)d_bs })
         if (p) (void)hipFree(p);
     if (e != hipSuccess) return set_err(h, RD_EHIP, std::string("pair emission: ") + hipGetErrorString(e));
     for (long k = 0; k < nout; ++k) { out[2 * k] = (int64_t)raw[k].x; out[2 * k + 1] = (int64_t)raw[k].y; }
+    if (count) *count = (long)total;
+    return RD_OK;
+}
+
+// ---- reaction candidates without allocation ----
+// rd_reaction_candidates_noalloc runs the kernels of rd_reaction_candidates
+// with their scratch memory (per-particle offsets, scan tile sums, the pair
+// list) in this file-scope device array instead of buffers allocated per
+// call.  Each GPU holds its own copy for the life of the process; its address
+// is taken once per device (rd_initialize) and kept in scratch_dev.  Every
+// kernel writes the scratch it later reads, so nothing needs clearing between
+// calls.  Calls that overlap on one GPU share the array and race.
+#define RD_NA_MAX_N (1 << 23)               // particles
+#define RD_NA_MAX_PAIRS (1 << 20)           // candidate pairs
+#define RD_NA_OFF_WORDS (RD_NA_MAX_N + 64)
+#define RD_NA_BSUM_WORDS (RD_NA_MAX_N / RD_SCAN_TILE + 64)
+#define RD_NA_WORDS (RD_NA_OFF_WORDS + RD_NA_BSUM_WORDS + 4 * RD_NA_MAX_PAIRS)
+static_assert((RD_NA_OFF_WORDS + RD_NA_BSUM_WORDS) % 4 == 0, "pair list is 16-byte aligned");
+static __device__ __align__(256) uint32_t scratch[RD_NA_WORDS];
+
+#define RD_NA_MAX_DEVICES 64
+static std::mutex scratch_mu;
+static std::atomic<void*> scratch_dev[RD_NA_MAX_DEVICES];
+
+// address of this device's scratch array (the device must be current)
+static int scratch_setup(rd_system* h)
+{
+    if (h->dev < 0 || h->dev >= RD_NA_MAX_DEVICES) return set_err(h, RD_EINVAL, "device index beyond 64");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_dev[h->dev].load()) return RD_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_err(h, RD_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_dev[h->dev].store(ptr);
+    return RD_OK;
+}
+
+extern "C" int rd_reaction_candidates_noalloc(rd_system* h, int64_t* out, long cap, long* count)
+{
+    if (!h || cap < 0 || (cap > 0 && !out)) return RD_EINVAL;
+    if (!h->initialized) return set_err(h, RD_ESTATE, "rd_initialize has not been called");
+    if (h->dirty || h->particles_dirty)
+        return set_err(h, RD_ESTATE, "the model or particles changed since the last upload: call rd_simulate "
+                                     "(or rd_reaction_candidates) first");
+    void* base = (h->dev >= 0 && h->dev < RD_NA_MAX_DEVICES) ? scratch_dev[h->dev].load() : nullptr;
+    if (!base) return set_err(h, RD_ESTATE, "no scratch address for this device");
+    float m2 = 0.0f;
+    for (int a = 0; a < h->nt * h->nt; ++a) m2 = std::max(m2, h->P.rr2[a]);
+    if (h->n == 0 || m2 <= 0.0f) { if (count) *count = 0; return RD_OK; }
+    if (h->n > RD_NA_MAX_N)
+        return set_err(h, RD_ERANGE, "more than 2^23 particles: the allocation-free scratch is too small; use "
+                                     "rd_reaction_candidates");
+    HIPCK(h, hipSetDevice(h->dev));
+    const int S = std::max(1, (int)std::ceil(std::sqrt((double)m2) * 1.0001 / h->cell));
+    const float* dtable = (const float*)((const char*)h->d_P + offsetof(RdDev, rr2));
+    uint32_t* off = (uint32_t*)base;
+    uint32_t* bsum = off + RD_NA_OFF_WORDS;
+    longlong2* pairs = reinterpret_cast<longlong2*>(bsum + RD_NA_BSUM_WORDS);
+    launch_pair_counts(h, dtable, S, off, bsum);
+    HIPCK(h, hipGetLastError());
+    uint32_t total = 0;
+    HIPCK(h, hipMemcpyAsync(&total, off + h->n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    if (total > (uint32_t)RD_NA_MAX_PAIRS)
+        return set_err(h, RD_ERANGE, "more than 2^20 candidate pairs: the allocation-free scratch is too small; "
+                                     "use rd_reaction_candidates");
+    if (total > 0) {
+        hipLaunchKernelGGL((k_pairs<true, longlong2>), dim3(wave_blocks(h)), dim3(RD_BLOCK), 0, h->stream, h->d_ps,
+                           h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, dtable, S, off, pairs);
+        HIPCK(h, hipGetLastError());
+        const long nout = std::min<long>((long)total, cap);
+        if (nout > 0)
+            HIPCK(h, hipMemcpyAsync(out, pairs, sizeof(longlong2) * (size_t)nout, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+    }
     if (count) *count = (long)total;
     return RD_OK;
 }

@@ -263,6 +263,17 @@ class Simulation:
     def reaction_candidates(self) -> np.ndarray:
         return self._pairs(self.lib.rd_reaction_candidates)
 
+    def reaction_candidates_noalloc(self, cap: int) -> np.ndarray:
+        """rd_reaction_candidates_noalloc: the same set, computed in the
+        library's fixed device scratch (no device allocation); at most `cap`
+        pairs are returned; the full count is kept in .last_count."""
+        cnt = C.c_long()
+        out = np.empty((max(cap, 1), 2), dtype=np.int64)
+        self._check(self.lib.rd_reaction_candidates_noalloc(self.h, _ptr(out), cap, C.byref(cnt)))
+        res = out[:min(cap, cnt.value)]
+        self.last_count = cnt.value
+        return res
+
     def checkpoint(self, path: str):
         """Binary snapshot: particles, replica layout, bonds, seed, step counter."""
         self._check(self.lib.rd_checkpoint_write(self.h, path.encode()))
