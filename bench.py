@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import dataclasses
 import json
 import os
 import sys
@@ -104,6 +105,57 @@ def bench_handle(sim, steps, warmup, tune_steps=0):
         raise RuntimeError(sim.lib.rd_last_error(sim.h).decode())
     sim.simulate(0)
     return out[0], out[2], out[3], out[4]
+
+
+def fp64_error(steps=20):
+    """FP32 GPU trajectory against the FP64 oracle on the 10k workload (same
+    start, same noise): the largest minimum-image position difference after
+    one step and after `steps` steps of a free run."""
+    import numpy as np
+    from oracle import oracle as orc
+    from readdygpu_amd import Simulation, workloads
+    spec, xyz, types = workloads.plain_10k()
+    o = orc.OracleSystem(spec)
+    L = np.asarray(spec.box_hi, float) - np.asarray(spec.box_lo, float)
+
+    def err(a, b):
+        d = np.abs(np.asarray(a, float) - np.asarray(b, float))
+        return float(np.minimum(d, L - d).max())
+
+    with Simulation(spec, xyz, types) as sim:
+        sim.simulate(1)
+        x1, _ = sim.copy_pos_from_device()
+        sim.simulate(steps - 1)
+        xk, _ = sim.copy_pos_from_device()
+    r1, b1 = o.run(xyz, types, 0, 1)
+    rk, bk = o.run(xyz, types, 0, steps)
+    res = {"workload": "plain_10k: 10,000 particles, 2 species, 100 nm box, harmonic repulsion, dt=0.1 ns; "
+                       "GPU FP32 vs oracle FP64 (oracle/rd_oracle.c orc_run), same start and noise",
+           "one_step_max_nm": err(x1, r1), "steps": steps, "free_run_max_nm": err(xk, rk),
+           "oracle_flags": int(b1 | bk)}
+    log(f"fp64 error: one step {res['one_step_max_nm']:.2e} nm, {steps} steps {res['free_run_max_nm']:.2e} nm")
+    return res
+
+
+def configured_run(spec, xyz, types, lc):
+    """The 1M workload as configured (500 steps from the initial state) with
+    the tuned cell size and launch shape: host wall time of the 500 steps as
+    one graph (rd_bench_steps pass 1), and the force-kernel average over the
+    500 steps after them."""
+    from readdygpu_amd import Simulation, workloads
+    steps = workloads.PLAIN_1M["steps"]
+    with Simulation(spec, xyz, types) as sim:
+        sim.set_launch_config(kernel=0, block=lc["used_block"], ppt=lc["used_ppt"])
+        out = (C.c_double * 5)()
+        rc = sim.lib.rd_bench_steps(sim.h, steps, out)
+        if rc != 0:
+            raise RuntimeError(sim.lib.rd_last_error(sim.h).decode())
+    n = len(types)
+    res = {"steps": steps, "value": n * steps / (out[0] / 1e3), "ms_per_step": out[0] / steps,
+           "window": f"steps 0-{steps} from the initial state (one graph, host clock)",
+           "force_avg_us_steps_500_1000": out[2] / steps * 1e3, "sort_avg_us_steps_500_1000": out[3] / steps * 1e3}
+    log(f"configured run: {steps} steps {res['ms_per_step'] * 1e3:.1f} us/step ({res['value']:.3e} p-steps/s)")
+    return res
 
 
 def bench_batched(steps=200, warmup=10):
@@ -606,11 +658,16 @@ def main():
         "traffic": read_traffic(),
         "kernel": "k_force_update",
         "kernel_avg_us": force_avg_s * 1e6,
+        "kernel_avg_from": f"dispatch timestamps of the event pass (steps {args.warmup + args.steps}-"
+                           f"{args.warmup + 2 * args.steps})",
         "bytes_alg_per_launch": B_ALG * n,
         "sort_avg_us": sort_ms / args.steps * 1e3,
     }
     passes = bench_passes(sim)
     sim.close()
+    cfg_spec = dataclasses.replace(spec, cell_size=info.cell_size)
+    full = configured_run(cfg_spec, xyz, types, lc)
+    fperr = fp64_error()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(spec, xyz, types)
@@ -646,7 +703,13 @@ def main():
             "force_kernel": {1: "gather"}.get(lc["used_kernel"], "?"),
             "workgroup": lc["used_block"],
             "particles_per_lane": lc["used_ppt"],
+            "timed_window": {"warmup": [0, args.warmup],
+                             "autotune": "trial steps on the warmed state; state and step counter restored",
+                             "graph_pass (value)": [args.warmup, args.warmup + args.steps],
+                             "event_pass (roofline)": [args.warmup + args.steps, args.warmup + 2 * args.steps]},
         },
+        "configured_run": full,
+        "fp64_error": fperr,
         "roofline": roofline,
         "passes": passes,
         "cpu_baseline": cpu,

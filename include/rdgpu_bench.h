@@ -12,12 +12,14 @@ extern "C" {
 #endif
 
 /* Run nsteps steps as one captured graph (timed on the host), then nsteps
- * more steps launched directly with HIP events around the force/update kernel
- * and the cell sort of every step (graph nodes carry no event timestamps).
+ * more steps launched directly with HIP events between the kernel groups of
+ * every step (graph nodes carry no event timestamps); the force/update
+ * kernel's events are timestamped by its own dispatch (hipExtLaunchKernelGGL).
  * out[0] = host wall time of graph launch + synchronize (ms),
- * out[1] = first event to last of pass 2 (ms), out[2] = sum of force/update kernel
- * times, out[3] = sum of cell-sort times, out[4] = sum of external-pass times
- * (composite or surrogate; 0 without).  out holds 5 values. */
+ * out[1] = first event to last of pass 2 (ms), out[2] = sum of the force/update
+ * kernel durations, out[3] = sum of cell-sort times (force kernel end to the
+ * event after the scatter), out[4] = sum of external-pass times (composite or
+ * surrogate; 0 without).  out holds 5 values. */
 int rd_bench_steps(rd_system* h, int nsteps, double* out);
 
 /* Copy `bytes` bytes device-to-device with a plain vector-copy kernel, reps

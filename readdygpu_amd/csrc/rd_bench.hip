@@ -30,7 +30,7 @@ extern "C" int rd_bench_steps(rd_system* h, int nsteps, double* out)
     if (!h || nsteps < 1 || !out) return RD_EINVAL;
     int rc = rd_host_refresh(h);
     if (rc) return rc;
-    std::vector<hipEvent_t> ev(4 * (size_t)nsteps, nullptr);
+    std::vector<hipEvent_t> ev(5 * (size_t)nsteps, nullptr);
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     double tf = 0.0, ts = 0.0, te = 0.0;
@@ -54,28 +54,29 @@ extern "C" int rd_bench_steps(rd_system* h, int nsteps, double* out)
     t1 = std::chrono::steady_clock::now();
     out[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
     h->step += (uint64_t)nsteps;
-    // pass 2: K more steps launched directly with events around each kernel
-    // group (events recorded inside a graph have no timestamps in HIP)
+    // pass 2: K more steps launched directly with events between the kernel
+    // groups (events recorded inside a graph have no timestamps in HIP); the
+    // force kernel's own events are timestamped by its dispatch
+    // (hipExtLaunchKernelGGL), so their interval is the kernel's duration
     for (int i = 0; i < nsteps; ++i) {
-        BCK(hipEventRecord(ev[4 * i], h->stream));
+        BCK(hipEventRecord(ev[5 * i], h->stream));
         if (ext) rd_host_launch_ext(h);
-        BCK(hipEventRecord(ev[4 * i + 1], h->stream));
-        rd_host_launch_force(h);
-        BCK(hipEventRecord(ev[4 * i + 2], h->stream));
+        BCK(hipEventRecord(ev[5 * i + 1], h->stream));
+        rd_host_launch_force(h, ev[5 * i + 2], ev[5 * i + 3]);
         rd_host_launch_sort(h, 1);
-        BCK(hipEventRecord(ev[4 * i + 3], h->stream));
+        BCK(hipEventRecord(ev[5 * i + 4], h->stream));
     }
     BCK(hipStreamSynchronize(h->stream));
     h->step += (uint64_t)nsteps;
     for (int i = 0; i < nsteps; ++i) {
-        BCK(hipEventElapsedTime(&ms, ev[4 * i], ev[4 * i + 1]));
+        BCK(hipEventElapsedTime(&ms, ev[5 * i], ev[5 * i + 1]));
         te += ms;
-        BCK(hipEventElapsedTime(&ms, ev[4 * i + 1], ev[4 * i + 2]));
+        BCK(hipEventElapsedTime(&ms, ev[5 * i + 2], ev[5 * i + 3]));
         tf += ms;
-        BCK(hipEventElapsedTime(&ms, ev[4 * i + 2], ev[4 * i + 3]));
+        BCK(hipEventElapsedTime(&ms, ev[5 * i + 3], ev[5 * i + 4]));
         ts += ms;
     }
-    BCK(hipEventElapsedTime(&ms, ev[0], ev[4 * (size_t)nsteps - 1]));
+    BCK(hipEventElapsedTime(&ms, ev[0], ev[5 * (size_t)nsteps - 1]));
     out[1] = ms;
     out[2] = tf;
     out[3] = ts;

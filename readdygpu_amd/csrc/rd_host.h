@@ -167,7 +167,7 @@ int rd_host_refresh(rd_system* h);
 int rd_host_set_step(rd_system* h, uint64_t step);
 int rd_host_set_state(rd_system* h, const std::vector<float>& xyz, const std::vector<int>& type);
 int rd_host_step_graph(rd_system* h, int k, hipGraphExec_t* exec);
-void rd_host_launch_force(rd_system* h);
+void rd_host_launch_force(rd_system* h, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void rd_host_launch_sort(rd_system* h, int advance);
 void rd_host_launch_ext(rd_system* h);
 int rd_host_ext_capacity(rd_system* h);

@@ -4,6 +4,7 @@
 // transfers, RDF.  A handle holds one system or a batch of independent
 // replicas; all replicas advance together through one set of kernels.
 #include "rd_host.h"
+#include <hip/hip_ext.h>
 #include "rd_kernels.h"
 
 #include <algorithm>
@@ -1130,38 +1131,56 @@ static void launch_sort(rd_system* h, int advance)
 
 static int wave_blocks(const rd_system* h) { return std::max(1, nblk(h->nwaves, RD_BLOCK / 64)); }
 
+// ev0/ev1 (measurement only): events the dispatch itself timestamps at the
+// kernel's start and end (hipExtLaunchKernelGGL), so their interval is the
+// kernel's own duration
 template <int B, int PPT>
-static void launch_force(rd_system* h)
+static void launch_force(rd_system* h, hipEvent_t ev0, hipEvent_t ev1)
 {
     const dim3 grid(std::max(1, nblk(h->nwaves, B / 64 * PPT)));
-    if constexpr (PPT == 1)
-        hipLaunchKernelGGL((k_force_update<B>), grid, dim3(B), 0, h->stream, h->d_ps, h->d_pn, h->d_kr, h->d_count,
-                           h->d_bsum, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1, h->d_st,
-                           h->d_noise, h->n);
-    else
-        hipLaunchKernelGGL((k_force_update_ppt<B, PPT>), grid, dim3(B), 0, h->stream, h->d_ps, h->d_pn, h->d_kr,
-                           h->d_count, h->d_bsum, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1,
-                           h->d_st, h->d_noise, h->n);
-}
-
-template <int B>
-static void launch_force_ppt(rd_system* h)
-{
-    switch (h->ppt) {
-    case 2: launch_force<B, 2>(h); break;
-    case 4: launch_force<B, 4>(h); break;
-    default: launch_force<B, 1>(h); break;
+    if constexpr (PPT == 1) {
+        if (ev0)
+            hipExtLaunchKernelGGL((k_force_update<B>), grid, dim3(B), 0, h->stream, ev0, ev1, 0,
+                                  (const float4*)h->d_ps, h->d_pn, h->d_kr, h->d_count, h->d_bsum,
+                                  (const uint32_t*)h->d_start, (const RdDev*)h->d_P, (const RdRep*)h->d_reps,
+                                  (const RdWave*)h->d_waves, h->nwaves, (const RdPot1Dev*)h->d_pot1, h->d_st,
+                                  (const float*)h->d_noise, h->n);
+        else
+            hipLaunchKernelGGL((k_force_update<B>), grid, dim3(B), 0, h->stream, h->d_ps, h->d_pn, h->d_kr,
+                               h->d_count, h->d_bsum, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1,
+                               h->d_st, h->d_noise, h->n);
+    } else {
+        if (ev0)
+            hipExtLaunchKernelGGL((k_force_update_ppt<B, PPT>), grid, dim3(B), 0, h->stream, ev0, ev1, 0,
+                                  (const float4*)h->d_ps, h->d_pn, h->d_kr, h->d_count, h->d_bsum,
+                                  (const uint32_t*)h->d_start, (const RdDev*)h->d_P, (const RdRep*)h->d_reps,
+                                  (const RdWave*)h->d_waves, h->nwaves, (const RdPot1Dev*)h->d_pot1, h->d_st,
+                                  (const float*)h->d_noise, h->n);
+        else
+            hipLaunchKernelGGL((k_force_update_ppt<B, PPT>), grid, dim3(B), 0, h->stream, h->d_ps, h->d_pn, h->d_kr,
+                               h->d_count, h->d_bsum, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves,
+                               h->d_pot1, h->d_st, h->d_noise, h->n);
     }
 }
 
-void rd_host_launch_force(rd_system* h)
+template <int B>
+static void launch_force_ppt(rd_system* h, hipEvent_t ev0, hipEvent_t ev1)
+{
+    switch (h->ppt) {
+    case 2: launch_force<B, 2>(h, ev0, ev1); break;
+    case 4: launch_force<B, 4>(h, ev0, ev1); break;
+    default: launch_force<B, 1>(h, ev0, ev1); break;
+    }
+}
+
+void rd_host_launch_force(rd_system* h, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (h->n <= 0) return;
     switch (h->block) {
-    case 64: launch_force_ppt<64>(h); break;
-    case 128: launch_force_ppt<128>(h); break;
-    case 512: launch_force_ppt<512>(h); break;
-    default: launch_force_ppt<256>(h); break;
+    case 64: launch_force_ppt<64>(h, ev0, ev1); break;
+    case 128: launch_force_ppt<128>(h, ev0, ev1); break;
+    case 512: launch_force_ppt<512>(h, ev0, ev1); break;
+    default: launch_force_ppt<256>(h, ev0, ev1); break;
     }
 }
 
