@@ -352,10 +352,14 @@ int rd_create_from_model(const rd_model* m, double max_cutoff, uint64_t seed, rd
  *      block, 32 per cell, CudaSimulation.cu:193, 2347).  Any configuration
  *      gives bitwise the same trajectory: pair gradients are summed in fixed
  *      point, so only speed changes. ---- */
-enum { RD_KERNEL_AUTO = 0, RD_KERNEL_GATHER = 1 };
+enum { RD_KERNEL_AUTO = 0, RD_KERNEL_GATHER = 1, RD_KERNEL_QUEUE = 2 };
 typedef struct {
     int kernel;            /* RD_KERNEL_*: GATHER one lane per particle walks its stencil rows,
-                              testing distances first and evaluating the listed hits after.
+                              testing distances first and evaluating the listed hits after, one
+                              workgroup per group of 64-slot work items.  QUEUE: the same per-lane
+                              work on a resident grid (occupancy x compute units) whose workgroups
+                              each own a contiguous share of the work items, taken by their waves
+                              from a ticket in LDS until none are left (ppt is ignored).
                               AUTO = GATHER */
     int used_kernel;       /* out: kernel in use */
     int block;             /* threads per workgroup of the force kernel: 64, 128, 256 or 512; 0 = 256 */
@@ -372,8 +376,8 @@ int rd_get_launch_config(rd_system* h, rd_launch_config* cfg);
  * per lane at the winning pair; keeps the fastest (the default shape, 256
  * threads and one particle per lane, unless another is more than 2% faster),
  * and restores the particle state and step counter, so tuning does not change
- * the run.  Values fixed by rd_set_launch_config (block, ppt) or the cell size
- * are kept. */
+ * the run.  Values fixed by rd_set_launch_config (kernel, block, ppt) or the
+ * cell size are kept; the work-queue form is tuned only when selected there. */
 int rd_autotune(rd_system* h, int trial_steps, double* best_ms_per_step);
 
 /* Lattice cell edge of a running system (<= 0: 1.01 x the largest interaction

@@ -134,6 +134,7 @@ struct rd_system {
     int kern = RD_KERNEL_GATHER;  // kernel in use (RD_KERNEL_*)
     int block = 256;              // force-kernel workgroup size in use
     int ppt = 1;                  // force-kernel particles per lane in use
+    int qgrid = 0, qgrid_block = 0; // resident workgroups of the work-queue form (for qgrid_block threads)
 
     // composite external potential (nearest piece) and its MLP surrogate
     std::vector<rd_pot1> comp;

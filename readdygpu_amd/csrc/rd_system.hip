@@ -101,6 +101,44 @@ __global__ __launch_bounds__(BLOCK) void k_force_update_ppt(RD_FORCE_ARGS)
     force_update<BLOCK, PPT>(ps, pn, kr, count, tsum, start, P, reps, waves, nwaves, pot1, st, noise, ntot);
 }
 
+// Resident-grid form: occupancy x CUs workgroups, each owning a contiguous
+// share of the 64-slot work items (shares in XCD order, as xcd_block maps
+// the per-item form, so one XCD's waves work on neighbouring items and share
+// its L2).  The waves of a workgroup take the share's items from a ticket in
+// LDS until none are left: a wave that finishes early takes the next item, as
+// the dispatcher would refill its slot, while the type-pair table is loaded
+// once per workgroup instead of once per 4 items.  (Tickets in global memory,
+// one per XCD, serialized ~5,000 atomics per address per step: 3.8x slower.)
+// Every wave leaves the loop when the ticket passes the share's end.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_force_update_queue(RD_FORCE_ARGS)
+{
+    __shared__ RdTab T;
+    __shared__ RdHitList il[BLOCK / 64];
+    __shared__ unsigned int ticket;
+    if (threadIdx.x == 0) ticket = 0u;
+    load_tab(T, P);                         // ends with a workgroup barrier
+    const int lane = threadIdx.x & 63;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int per = (nwaves + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int lo = b * per, hi = min(nwaves, lo + per);
+    for (;;) {
+        int w = 0;
+        if (lane == 0) w = lo + (int)atomicAdd(&ticket, 1u);
+        w = __builtin_amdgcn_readfirstlane(__shfl(w, 0, 64));
+        if (w >= hi) break;
+        const RdWave it = waves[w];
+        const uint32_t s = (uint32_t)(it.s0 + lane);
+        if (s < (uint32_t)it.s1) {
+            const RdRep R = reps[it.rep];
+            const float4 p = ps[s];
+            bool sat;
+            const float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &sat, &il[threadIdx.x >> 6]);
+            em_update_store(P, R, T, st, noise, ntot, s, p, g, sat, pn, kr, count, tsum);
+        }
+    }
+}
+
 // gradient + energy (FP64 accumulation) at the current positions
 __global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict__ ps,
                                                         const uint32_t* __restrict__ start,
@@ -1163,9 +1201,44 @@ static void launch_force(rd_system* h, hipEvent_t ev0, hipEvent_t ev1)
     }
 }
 
+// resident workgroups of the work-queue form, at most one per work-item group
+template <int B>
+static int queue_grid(rd_system* h)
+{
+    if (h->qgrid_block != B) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_force_update_queue<B>, B, 0) !=
+                hipSuccess || per_cu < 1)
+            per_cu = 1;
+        h->qgrid = per_cu * h->ncu;
+        h->qgrid_block = B;
+    }
+    return std::max(1, std::min(h->qgrid, nblk(h->nwaves, B / 64)));
+}
+
+template <int B>
+static void launch_force_queue(rd_system* h, hipEvent_t ev0, hipEvent_t ev1)
+{
+    const dim3 grid(queue_grid<B>(h));
+    if (ev0)
+        hipExtLaunchKernelGGL((k_force_update_queue<B>), grid, dim3(B), 0, h->stream, ev0, ev1, 0,
+                              (const float4*)h->d_ps, h->d_pn, h->d_kr, h->d_count, h->d_bsum,
+                              (const uint32_t*)h->d_start, (const RdDev*)h->d_P, (const RdRep*)h->d_reps,
+                              (const RdWave*)h->d_waves, h->nwaves, (const RdPot1Dev*)h->d_pot1, h->d_st,
+                              (const float*)h->d_noise, h->n);
+    else
+        hipLaunchKernelGGL((k_force_update_queue<B>), grid, dim3(B), 0, h->stream, h->d_ps, h->d_pn, h->d_kr,
+                           h->d_count, h->d_bsum, h->d_start, h->d_P, h->d_reps, h->d_waves, h->nwaves, h->d_pot1,
+                           h->d_st, h->d_noise, h->n);
+}
+
 template <int B>
 static void launch_force_ppt(rd_system* h, hipEvent_t ev0, hipEvent_t ev1)
 {
+    if (h->kern == RD_KERNEL_QUEUE) {
+        launch_force_queue<B>(h, ev0, ev1);
+        return;
+    }
     switch (h->ppt) {
     case 2: launch_force<B, 2>(h, ev0, ev1); break;
     case 4: launch_force<B, 4>(h, ev0, ev1); break;
@@ -1255,7 +1328,7 @@ static int upload(rd_system* h)
     if (rc) return rc;
     h->P.gext = h->ext_active() ? h->d_gext : nullptr;
     h->P.ext_mask = h->comp_mask;
-    h->kern = RD_KERNEL_GATHER;
+    h->kern = h->lc.kernel == RD_KERNEL_QUEUE ? RD_KERNEL_QUEUE : RD_KERNEL_GATHER;
     h->block = (h->lc.block == 64 || h->lc.block == 128 || h->lc.block == 512) ? h->lc.block : 256;
     h->ppt = (h->lc.ppt == 2 || h->lc.ppt == 4) ? h->lc.ppt : 1;
     HIPCK(h, hipMemcpyAsync(h->d_P, &h->P, sizeof(RdDev), hipMemcpyHostToDevice, h->stream));
@@ -1411,7 +1484,7 @@ int rd_host_set_state(rd_system* h, const std::vector<float>& xyz, const std::ve
 extern "C" int rd_set_launch_config(rd_system* h, const rd_launch_config* cfg)
 {
     if (!h || !cfg) return RD_EINVAL;
-    if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_GATHER) return set_err(h, RD_EINVAL, "bad kernel");
+    if (cfg->kernel < RD_KERNEL_AUTO || cfg->kernel > RD_KERNEL_QUEUE) return set_err(h, RD_EINVAL, "bad kernel");
     if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256 && cfg->block != 512)
         return set_err(h, RD_EINVAL, "workgroup size must be 64, 128, 256 or 512");
     if (cfg->ppt != 0 && cfg->ppt != 1 && cfg->ppt != 2 && cfg->ppt != 4)

@@ -333,9 +333,11 @@ class Simulation:
             self._check(self.lib.rd_apply_reactions_replicas(self.h, _ptr(cr), _ptr(rr)))
 
     def set_launch_config(self, kernel: int = 0, block: int = 0, ppt: int = 0):
-        """Force kernel (0 automatic, 1 per-particle gather), its workgroup
-        size (64, 128, 256, 512; 0 = 256) and particles per lane (1, 2, 4;
-        0 = 1)."""
+        """Force kernel (0 automatic, 1 per-particle gather with one workgroup
+        per group of 64-slot work items, 2 the same gather on a resident grid
+        whose workgroups take their share of the work items from LDS tickets), its workgroup size (64, 128,
+        256, 512; 0 = 256) and particles per lane (1, 2, 4; 0 = 1; the queue
+        form takes one)."""
         c = _lib.rd_launch_config()
         c.kernel = int(kernel)
         c.block = int(block)
@@ -348,7 +350,8 @@ class Simulation:
         return {k: getattr(c, k) for k, _ in c._fields_}
 
     def autotune(self, trial_steps: int = 20, cache: Optional[str] = None) -> float:
-        """Pick the cell size, workgroup size and particles per lane by timing;
+        """Pick the cell size, force-kernel form, workgroup size and particles
+        per lane by timing;
         state is restored.  ``cache`` (default: $RDGPU_TUNE_CACHE) names a JSON
         file of earlier choices keyed by the device and the workload (spec,
         particle count, replica layout, bonds, composite): a hit applies the
@@ -365,7 +368,8 @@ class Simulation:
                 db = {}
             hit = db.get(key) if isinstance(db, dict) else None
             if hit:
-                self.set_launch_config(kernel=1, block=int(hit["block"]), ppt=int(hit["ppt"]))
+                self.set_launch_config(kernel=int(hit.get("kernel", 1)), block=int(hit["block"]),
+                                       ppt=int(hit["ppt"]))
                 self._check(self.lib.rd_set_cell_size(self.h, float(hit["cell_size"])))
                 return float(hit["ms_per_step"])
         ms = C.c_double()
@@ -373,8 +377,8 @@ class Simulation:
         if path:
             lc = self.launch_config()
             db = db if isinstance(db, dict) else {}
-            db[key] = {"cell_size": self.info().cell_size, "block": lc["used_block"], "ppt": lc["used_ppt"],
-                       "ms_per_step": ms.value}
+            db[key] = {"cell_size": self.info().cell_size, "kernel": lc["used_kernel"], "block": lc["used_block"],
+                       "ppt": lc["used_ppt"], "ms_per_step": ms.value}
             tmp = path + ".tmp"
             with open(tmp, "w") as f:
                 json.dump(db, f, indent=1, sort_keys=True)

@@ -157,8 +157,8 @@ def test_step_is_deterministic_and_noise_array_equals_generator(c1):
 @pytest.mark.parametrize("which", ["plain_10k", "rich", "plain_1m_slice", "dense"])
 def test_launch_configs_give_identical_trajectories(which, c1):
     """Three cell sizes (different sort orders inside cells), workgroup sizes,
-    particles per lane and the autotuner: bitwise the same positions and
-    gradients (fixed-point pair sums)."""
+    particles per lane, the work-queue form and the autotuner: bitwise the
+    same positions and gradients (fixed-point pair sums)."""
     if which == "plain_10k":
         spec, xyz, types = c1
     elif which == "rich":
@@ -170,7 +170,8 @@ def test_launch_configs_give_identical_trajectories(which, c1):
     ref = None
     rc = float(spec.pair_range().max())
     shapes = [dict(kernel=1), dict(kernel=0, block=64), dict(kernel=1, block=512), dict(kernel=1, block=128),
-              dict(kernel=1, block=128, ppt=2), dict(kernel=1, ppt=4), dict(kernel=0, block=64, ppt=2)]
+              dict(kernel=1, block=128, ppt=2), dict(kernel=1, ppt=4), dict(kernel=0, block=64, ppt=2),
+              dict(kernel=2), dict(kernel=2, block=64), dict(kernel=2, block=512)]
     for cell in (0.0, 1.3 * rc, 2.1 * rc):
         for shape in shapes:
             spec.cell_size = cell
@@ -180,7 +181,8 @@ def test_launch_configs_give_identical_trajectories(which, c1):
                 x, _ = sim.copy_pos_from_device()
                 g, _ = sim.gradients()
                 used = sim.launch_config()
-            assert used["used_kernel"] == 1 and used["used_block"] == (shape.get("block") or 256), used
+            assert used["used_kernel"] == (shape.get("kernel") or 1), used
+            assert used["used_block"] == (shape.get("block") or 256), used
             assert used["used_ppt"] == (shape.get("ppt") or 1), used
             if ref is None:
                 ref = (x, g)

@@ -34,9 +34,12 @@ def batch():
     return small_batch()
 
 
-def test_replicas_equal_separate_runs_bitwise(batch):
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_replicas_equal_separate_runs_bitwise(batch, kernel):
+    """Per-workgroup items (1) and the resident-grid work queue (2)."""
     spec, boxes, counts, xyz, types = batch
     with Simulation(spec, xyz, types, replicas=(boxes, counts)) as sim:
+        sim.set_launch_config(kernel=kernel)
         info = sim.info()
         assert info.n_replicas == len(counts) and info.n == int(counts.sum())
         sim.simulate(25)
@@ -67,7 +70,7 @@ def test_replica_noise_and_cell_ids_bitexact(batch):
         assert np.array_equal(ids[off:off + n], o.cell_index(x[off:off + n])[0]), f"replica {r}"
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1, 2])
 def test_replica_gradients_and_pairs(batch, kernel):
     spec, boxes, counts, xyz, types = batch
     with Simulation(spec, xyz, types, replicas=(boxes, counts)) as sim:
