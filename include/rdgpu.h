@@ -271,6 +271,13 @@ int rd_check_invariants(rd_system* h, long* violations);
  * gradient before the Euler-Maruyama update. */
 enum { RD_EXT_EXACT = 0, RD_EXT_SURROGATE = 1 };
 int rd_set_composite(rd_system* h, int n_pieces, const rd_pot1* pieces, uint32_t type_mask);
+/* How the pieces combine: RD_COMBINE_NEAREST (default) as above, or
+ * RD_COMBINE_SUM: U = sum_i U_i and the gradient is the sum of the pieces'
+ * gradients, as the reference's orderOne adds every order-one potential
+ * (CudaSimulation.cu:1049-1236).  Applies to the exact pass, rd_composite_eval
+ * and the surrogate's training data. */
+enum { RD_COMBINE_NEAREST = 0, RD_COMBINE_SUM = 1 };
+int rd_set_composite_combine(rd_system* h, int combine);
 int rd_set_external_mode(rd_system* h, int mode);
 /* exact composite at m host positions: grad[3m], energy[m] (either may be NULL) */
 int rd_composite_eval(rd_system* h, long m, const float* xyz, float* grad, float* energy);

@@ -27,6 +27,12 @@ int rd_bench_steps(rd_system* h, int nsteps, double* out);
  * ms: average kernel time. */
 int rd_bench_copy(rd_system* h, long bytes, int reps, double* ms);
 
+/* The external pass alone (exact composite or surrogate, whichever the step
+ * uses) on the current state, reps launches; ms: average.  The positions do
+ * not move, so this times a pass that the step itself could not run, e.g. a
+ * composite whose step diverges. */
+int rd_bench_ext(rd_system* h, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,7 @@ def lib():
             "orc_rdf_normalize": (None, [C.c_int, C.c_int, C.c_double, C.c_double, V, C.c_long, V, V]),
             "orc_piece_eval": (C.c_double, [V, V, V]),
             "orc_composite_eval": (None, [V, C.c_int, C.c_long, V, V, V, V, V, C.c_int]),
+            "orc_composite_sum": (None, [V, C.c_int, C.c_long, V, V, V, C.c_int]),
             "orc_bf16": (C.c_float, [C.c_float]),
             "orc_mlp_forward": (None, [V, C.c_long, V, C.c_int, V]),
             "orc_mlp_loss_grad": (C.c_double, [V, C.c_long, V, V, C.c_int, V]),
@@ -276,6 +277,17 @@ def composite(pieces, xyz, nthreads=0):
     lib().orc_composite_eval(C.cast(arr, C.c_void_p), len(pieces), m, _p(x), _p(g), _p(e), _p(w), _p(gap),
                              int(nthreads))
     return g, e, w, gap
+
+
+def composite_sum(pieces, xyz, nthreads=0):
+    """Sum of the pieces at FP64: (grad [m,3], energy [m])."""
+    arr = pieces_array(pieces)
+    x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    m = len(x)
+    g = np.zeros((m, 3))
+    e = np.zeros(m)
+    lib().orc_composite_sum(C.cast(arr, C.c_void_p), len(pieces), m, _p(x), _p(g), _p(e), int(nthreads))
+    return g, e
 
 
 def piece_eval(piece, x):

@@ -49,6 +49,28 @@ void orc_composite_eval(const orc_pot1* pieces, int npieces, long m, const doubl
     }
 }
 
+/* Sum of the pieces (CudaSimulation.cu:1049-1236 adds every order-one
+ * potential of a particle's type): energy and gradient per position. */
+void orc_composite_sum(const orc_pot1* pieces, int npieces, long m, const double* xyz, double* grad,
+                       double* energy, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+    for (long i = 0; i < m; ++i) {
+        const double* x = xyz + 3 * i;
+        double e = 0.0, gs[3] = { 0, 0, 0 };
+        for (int k = 0; k < npieces; ++k) {
+            double g[3];
+            e += orc_piece_eval(&pieces[k], x, g);
+            gs[0] += g[0]; gs[1] += g[1]; gs[2] += g[2];
+        }
+        if (grad) { grad[3 * i] = gs[0]; grad[3 * i + 1] = gs[1]; grad[3 * i + 2] = gs[2]; }
+        if (energy) energy[i] = e;
+    }
+}
+
 float orc_bf16(float v)
 {
     uint32_t u;

@@ -44,6 +44,9 @@ double orc_piece_eval(const orc_pot1* p, const double x[3], double g[3]);
  * gap[i] = second-smallest minus smallest piece energy (NULL to skip) */
 void orc_composite_eval(const orc_pot1* pieces, int npieces, long m, const double* xyz, double* grad,
                         double* energy, int* which, double* gap, int nthreads);
+/* sum of the pieces: energy[m] and gradient[3m] (either may be NULL) */
+void orc_composite_sum(const orc_pot1* pieces, int npieces, long m, const double* xyz, double* grad,
+                       double* energy, int nthreads);
 
 /* round an FP32 value to BF16 (nearest, ties to even), returned as FP32 */
 float orc_bf16(float v);

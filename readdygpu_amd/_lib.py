@@ -168,6 +168,7 @@ _SIGS = {
     "rd_model_bonds": (C.c_int, [P, P, P, P, P]),
     "rd_create_from_model": (C.c_int, [P, C.c_double, C.c_uint64, C.POINTER(P)]),
     "rd_set_composite": (C.c_int, [P, C.c_int, P, C.c_uint32]),
+    "rd_set_composite_combine": (C.c_int, [P, C.c_int]),
     "rd_set_external_mode": (C.c_int, [P, C.c_int]),
     "rd_composite_eval": (C.c_int, [P, C.c_long, P, P, P]),
     "rd_surrogate_init": (C.c_int, [P, C.c_uint64]),
@@ -182,6 +183,7 @@ _SIGS = {
     "rd_surrogate_validate": (C.c_int, [P, C.c_long, C.c_uint64, C.POINTER(C.c_double)]),
     "rd_bench_steps": (C.c_int, [P, C.c_int, P]),
     "rd_bench_copy": (C.c_int, [P, C.c_long, C.c_int, C.POINTER(C.c_double)]),
+    "rd_bench_ext": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
 }
 
 _lib = None

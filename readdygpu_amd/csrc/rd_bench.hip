@@ -117,3 +117,30 @@ done:
     if (e1) (void)hipEventDestroy(e1);
     return rc;
 }
+
+extern "C" int rd_bench_ext(rd_system* h, int reps, double* avg_ms)
+{
+    if (!h || reps < 1 || !avg_ms) return RD_EINVAL;
+    int rc = rd_host_refresh(h);
+    if (rc) return rc;
+    if (!h->ext_active()) {
+        h->err = "no external pass (set a composite first)";
+        return RD_ESTATE;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    BCK(hipEventCreate(&e0));
+    BCK(hipEventCreate(&e1));
+    rd_host_launch_ext(h);                   // warm-up
+    BCK(hipEventRecord(e0, h->stream));
+    for (int r = 0; r < reps; ++r) rd_host_launch_ext(h);
+    BCK(hipEventRecord(e1, h->stream));
+    BCK(hipEventSynchronize(e1));
+    BCK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
+}
+

@@ -139,6 +139,7 @@ struct rd_system {
     // composite external potential (nearest piece) and its MLP surrogate
     std::vector<rd_pot1> comp;
     uint32_t comp_mask = 0;
+    bool comp_sum = false;           // RD_COMBINE_SUM: sum of the pieces (else nearest piece)
     int npieces = 0;
     void* d_pieces = nullptr;
     int ext_mode = RD_EXT_EXACT;

@@ -2,7 +2,8 @@
 //
 // The composite is a geometry of order-one pieces (attractive disks and
 // cylinders, the forms of CudaSimulation.cu:1049-1148 with any normal); a
-// particle feels the piece nearest to it in energy.  Evaluating it exactly
+// particle feels the piece nearest to it in energy, or the sum of all pieces
+// (rd_set_composite_combine).  Evaluating it exactly
 // costs one pass over every piece per particle (the reference's orderOne loop
 // shape, CudaSimulation.cu:1049-1236), VALU-bound.  The surrogate is an MLP
 // 3 -> 64 -> 64 -> 64 -> 3 trained on the GPU, by the kernels below, on
@@ -68,9 +69,21 @@ __device__ __forceinline__ float3 piece_grad(const RdPiece& p, float x, float y,
 
 #define RD_PIECES_MAX 2048
 
-// nearest piece for one position: energy and gradient (first argmin)
+// composite at one position: energy and gradient.  SUM: the sum over every
+// piece, as orderOne adds every order-one potential (CudaSimulation.cu:
+// 1049-1236); otherwise the nearest piece in energy (first argmin).
+template <bool SUM>
 __device__ __forceinline__ float4 composite_at(const RdPiece* __restrict__ pc, int np, float x, float y, float z)
 {
+    if constexpr (SUM) {
+        float gx = 0.f, gy = 0.f, gz = 0.f, e = 0.f;
+        for (int k = 0; k < np; ++k) {
+            const float3 g = piece_grad(pc[k], x, y, z);
+            gx += g.x; gy += g.y; gz += g.z;
+            e += piece_energy(pc[k], x, y, z);
+        }
+        return make_float4(gx, gy, gz, e);
+    }
     float best = INFINITY;
     int kb = 0;
     for (int k = 0; k < np; ++k) {
@@ -90,6 +103,7 @@ __device__ __forceinline__ float4 composite_at(const RdPiece* __restrict__ pc, i
 // into an LDS queue; whenever 256 are queued, every thread evaluates one, so
 // no lane idles on an unselected particle.
 // Piece table in dynamic LDS: np * sizeof(RdPiece) bytes.
+template <bool SUM>
 __global__ __launch_bounds__(256) void k_composite_slots(const float4* __restrict__ ps, int n,
                                                          const RdPiece* __restrict__ pieces, int np,
                                                          uint32_t type_mask, float4* __restrict__ gext)
@@ -101,7 +115,7 @@ __global__ __launch_bounds__(256) void k_composite_slots(const float4* __restric
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     auto eval = [&](uint32_t s) {
         float4 p = ps[s];
-        gext[s] = composite_at(pc, np, p.x, p.y, p.z);
+        gext[s] = composite_at<SUM>(pc, np, p.x, p.y, p.z);
     };
     int qn = 0;                       // queued entries (same in every thread)
     for (long base = (long)xcd_block(blockIdx.x, gridDim.x) * 256; base < n; base += (long)gridDim.x * 256) {
@@ -131,6 +145,7 @@ __global__ __launch_bounds__(256) void k_composite_slots(const float4* __restric
 }
 
 // exact pass when the composite acts on every type: one slot per thread
+template <bool SUM>
 __global__ __launch_bounds__(256) void k_composite_all(const float4* __restrict__ ps, int n,
                                                        const RdPiece* __restrict__ pieces, int np,
                                                        float4* __restrict__ gext)
@@ -141,10 +156,11 @@ __global__ __launch_bounds__(256) void k_composite_all(const float4* __restrict_
     const long s = (long)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (s >= n) return;
     const float4 p = ps[s];
-    gext[s] = composite_at(pc, np, p.x, p.y, p.z);
+    gext[s] = composite_at<SUM>(pc, np, p.x, p.y, p.z);
 }
 
 // exact evaluation at given points (float4 input, .w ignored)
+template <bool SUM>
 __global__ __launch_bounds__(256) void k_composite_points(const float4* __restrict__ x, long m,
                                                           const RdPiece* __restrict__ pieces, int np,
                                                           float4* __restrict__ out)
@@ -155,7 +171,7 @@ __global__ __launch_bounds__(256) void k_composite_points(const float4* __restri
     long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
     float4 p = x[i];
-    out[i] = composite_at(pc, np, p.x, p.y, p.z);
+    out[i] = composite_at<SUM>(pc, np, p.x, p.y, p.z);
 }
 
 // ===========================================================================
@@ -165,6 +181,7 @@ __device__ __forceinline__ float u01(uint32_t b) { return (float)(b >> 8) * (1.0
 
 __device__ __forceinline__ float norm_coord(float x, float lo, float sc) { return (x - lo) * sc - 1.0f; }
 
+template <bool SUM>
 __global__ __launch_bounds__(256) void k_sur_data(long m, uint64_t seed, RdMlpNorm nm, float span0, float span1,
                                                   float span2, const RdPiece* __restrict__ pieces, int np,
                                                   float4* __restrict__ xn, float4* __restrict__ tg,
@@ -182,7 +199,7 @@ __global__ __launch_bounds__(256) void k_sur_data(long m, uint64_t seed, RdMlpNo
         float x = nm.lo[0] + u01(c[0]) * span0;
         float y = nm.lo[1] + u01(c[1]) * span1;
         float z = nm.lo[2] + u01(c[2]) * span2;
-        float4 g = composite_at(pc, np, x, y, z);
+        float4 g = composite_at<SUM>(pc, np, x, y, z);
         xn[i] = make_float4(norm_coord(x, nm.lo[0], nm.sc[0]), norm_coord(y, nm.lo[1], nm.sc[1]),
                             norm_coord(z, nm.lo[2], nm.sc[2]), 0.0f);
         tg[i] = make_float4(g.x, g.y, g.z, 0.0f);
@@ -762,12 +779,14 @@ void rd_host_launch_ext(rd_system* h)
         hipLaunchKernelGGL(k_mlp_slots, dim3(blocks), dim3(256), 0, h->stream, (const RdMlpPack*)h->d_pack,
                            (const float4*)h->d_ps, h->n, h->mlp_norm, h->comp_mask, h->d_gext);
     } else if ((h->comp_mask & all_types) == all_types) {
-        hipLaunchKernelGGL(k_composite_all, dim3(nb256(h->n)), dim3(256), pieces_lds(h), h->stream, h->d_ps, h->n,
-                           (const RdPiece*)h->d_pieces, h->npieces, h->d_gext);
+        hipLaunchKernelGGL(h->comp_sum ? k_composite_all<true> : k_composite_all<false>, dim3(nb256(h->n)), dim3(256),
+                           pieces_lds(h), h->stream, h->d_ps, h->n, (const RdPiece*)h->d_pieces, h->npieces,
+                           h->d_gext);
     } else {
         unsigned blocks = std::min(h->ext_blocks[0] > 0 ? (unsigned)h->ext_blocks[0] : 2048u, nb256(h->n));
-        hipLaunchKernelGGL(k_composite_slots, dim3(blocks), dim3(256), pieces_lds(h), h->stream, h->d_ps, h->n,
-                           (const RdPiece*)h->d_pieces, h->npieces, h->comp_mask, h->d_gext);
+        hipLaunchKernelGGL(h->comp_sum ? k_composite_slots<true> : k_composite_slots<false>, dim3(blocks), dim3(256),
+                           pieces_lds(h), h->stream, h->d_ps, h->n, (const RdPiece*)h->d_pieces, h->npieces,
+                           h->comp_mask, h->d_gext);
     }
 }
 
@@ -777,7 +796,8 @@ int rd_host_ext_capacity(rd_system* h)
 {
     if (!h->ext_active()) return RD_OK;
     int per = 0;
-    SHIPCK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_composite_slots, 256, pieces_lds(h)));
+    SHIPCK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_composite_slots<false>, 256,
+                                                           pieces_lds(h)));
     h->ext_blocks[0] = std::max(per, 1) * h->ncu;
     SHIPCK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_mlp_slots, 256, 0));
     h->ext_blocks[1] = std::max(per, 1) * h->ncu;
@@ -813,10 +833,11 @@ extern "C" int rd_set_composite(rd_system* h, int n_pieces, const rd_pot1* piece
     static bool attr = false;
     if (!attr) {
         const int bytes = (int)(sizeof(RdPiece) * RD_PIECES_MAX);
-        SHIPCK(h, hipFuncSetAttribute((const void*)k_composite_slots, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        SHIPCK(h, hipFuncSetAttribute((const void*)k_composite_all, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        SHIPCK(h, hipFuncSetAttribute((const void*)k_composite_points, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        SHIPCK(h, hipFuncSetAttribute((const void*)k_sur_data, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        const void* fns[] = { (const void*)k_composite_slots<false>, (const void*)k_composite_slots<true>,
+                              (const void*)k_composite_all<false>, (const void*)k_composite_all<true>,
+                              (const void*)k_composite_points<false>, (const void*)k_composite_points<true>,
+                              (const void*)k_sur_data<false>, (const void*)k_sur_data<true> };
+        for (const void* f : fns) SHIPCK(h, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         attr = true;
     }
     for (int i = 0; i < n_pieces; ++i) {
@@ -836,6 +857,14 @@ extern "C" int rd_set_composite(rd_system* h, int n_pieces, const rd_pot1* piece
     h->npieces = n_pieces;
     h->comp_mask = type_mask;
     h->comp.assign(pieces, pieces + n_pieces);
+    h->dirty = true;
+    return RD_OK;
+}
+
+extern "C" int rd_set_composite_combine(rd_system* h, int combine)
+{
+    if (!h || (combine != RD_COMBINE_NEAREST && combine != RD_COMBINE_SUM)) return RD_EINVAL;
+    h->comp_sum = combine == RD_COMBINE_SUM;
     h->dirty = true;
     return RD_OK;
 }
@@ -862,8 +891,9 @@ extern "C" int rd_composite_eval(rd_system* h, long m, const float* xyz, float* 
     hipError_t e = hipMalloc(&dout, sizeof(float4) * m);
     if (e == hipSuccess) e = hipMemcpyAsync(dx, x.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_composite_points, dim3(nb256(m)), dim3(256), pieces_lds(h), h->stream, dx, m,
-                           (const RdPiece*)h->d_pieces, h->npieces, dout);
+        hipLaunchKernelGGL(h->comp_sum ? k_composite_points<true> : k_composite_points<false>, dim3(nb256(m)),
+                           dim3(256), pieces_lds(h), h->stream, dx, m, (const RdPiece*)h->d_pieces, h->npieces,
+                           dout);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(o.data(), dout, sizeof(float4) * m, hipMemcpyDeviceToHost, h->stream);
@@ -1010,8 +1040,9 @@ extern "C" int rd_surrogate_make_data(rd_system* h, long m, uint64_t seed, doubl
     const RdMlpNorm& nm = h->mlp_norm;
     float span[3];
     for (int d = 0; d < 3; ++d) span[d] = (float)(h->norm_hi[d] - nm.lo[d]);
-    hipLaunchKernelGGL(k_sur_data, dim3(nb256(m)), dim3(256), pieces_lds(h), h->stream, m, seed, nm, span[0], span[1], span[2],
-                       (const RdPiece*)h->d_pieces, h->npieces, h->d_xn, h->d_tg, h->d_sbsum);
+    hipLaunchKernelGGL(h->comp_sum ? k_sur_data<true> : k_sur_data<false>, dim3(nb256(m)), dim3(256), pieces_lds(h),
+                       h->stream, m, seed, nm, span[0], span[1], span[2], (const RdPiece*)h->d_pieces, h->npieces,
+                       h->d_xn, h->d_tg, h->d_sbsum);
     hipLaunchKernelGGL(k_sur_scale_sum, dim3(1), dim3(256), 0, h->stream, h->d_sbsum, (int)nb256(m), m, h->d_gscale);
     hipLaunchKernelGGL(k_sur_normalise, dim3(nb256(m)), dim3(256), 0, h->stream, h->d_tg, m, h->d_gscale);
     SHIPCK(h, hipGetLastError());
@@ -1177,8 +1208,9 @@ extern "C" int rd_surrogate_validate(rd_system* h, long m, uint64_t seed, double
     const RdMlpNorm& nm = h->mlp_norm;
     float span[3];
     for (int d = 0; d < 3; ++d) span[d] = (float)(h->norm_hi[d] - nm.lo[d]);
-    hipLaunchKernelGGL(k_sur_data, dim3(nb256(m)), dim3(256), pieces_lds(h), h->stream, m, seed, nm, span[0], span[1], span[2],
-                       (const RdPiece*)h->d_pieces, h->npieces, dx, dt, bs);
+    hipLaunchKernelGGL(h->comp_sum ? k_sur_data<true> : k_sur_data<false>, dim3(nb256(m)), dim3(256), pieces_lds(h),
+                       h->stream, m, seed, nm, span[0], span[1], span[2], (const RdPiece*)h->d_pieces, h->npieces, dx,
+                       dt, bs);
     hipLaunchKernelGGL(k_mlp_points, dim3(mlp_point_blocks(m)), dim3(256), 0, h->stream, (const RdMlpPack*)h->d_pack,
                        (const float4*)dx, m, dy);
     hipError_t e = hipGetLastError();

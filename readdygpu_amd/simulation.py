@@ -422,9 +422,11 @@ class Simulation:
         return f"{buf.value.decode()}|{h.hexdigest()}"
 
     # -- composite external potential and surrogate (no reference counterpart) --
-    def set_composite(self, pieces, types=None):
-        """Nearest-piece composite of attractive DISK / CYLINDER pieces
-        (model.OrderOnePotential); applies to `types` (default: all)."""
+    def set_composite(self, pieces, types=None, combine: str = "nearest"):
+        """Composite of attractive DISK / CYLINDER pieces
+        (model.OrderOnePotential), combined by nearest piece or, with
+        combine="sum", summed as orderOne sums every order-one potential;
+        applies to `types` (default: all)."""
         arr = (_lib.rd_pot1 * max(len(pieces), 1))()
         for i, p in enumerate(pieces):
             q = arr[i]
@@ -437,7 +439,10 @@ class Simulation:
             q.type_mask = 0xFFFFFFFF
         mask = 0xFFFFFFFF if types is None else sum(1 << int(t) for t in types)
         self._check(self.lib.rd_set_composite(self.h, len(pieces), C.cast(arr, C.c_void_p), mask))
-        self._tune_tags["composite"] = [len(pieces), mask]
+        if combine not in ("nearest", "sum"):
+            raise ValueError("combine is 'nearest' or 'sum'")
+        self._check(self.lib.rd_set_composite_combine(self.h, 1 if combine == "sum" else 0))
+        self._tune_tags["composite"] = [len(pieces), mask, combine]
 
     def set_external_mode(self, surrogate: bool):
         self._check(self.lib.rd_set_external_mode(self.h, 1 if surrogate else 0))
