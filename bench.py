@@ -318,6 +318,35 @@ def summed_slab(spec, pieces, xyz, types, samples, train_steps):
     return res
 
 
+def bench_capacity(samples=4_000_000):
+    """Where the 3-64-64-64-3 surrogate's accuracy runs out, on the learned
+    slab's box (nearest-piece composites of its first P pieces): relative force
+    RMSE on fresh samples after 3,000 optimiser steps for P = 1 ... 512, and
+    for P = 512 after 10,000 and 30,000 steps (more training does not close
+    the gap: the error is set by the network's capacity against the number of
+    switch surfaces between pieces)."""
+    from readdygpu_amd import Simulation, workloads
+    spec, pieces, xyz, types = workloads.learned_slab(n=20_000)
+    res = {"workload": "learned_slab box (2000x2000x400 nm), nearest-piece composite of the first P of its 512 "
+                       "pieces; 3-64-64-64-3 BF16 MLP, batch 8192, lr 3e-3 cosine to 1e-5, 4M samples",
+           "pieces": {}, "training_steps_512": {}}
+    with Simulation(spec, xyz, types) as sim:
+        for p, steps in [(1, 3000), (4, 3000), (16, 3000), (64, 3000), (512, 3000), (512, 10000), (512, 30000)]:
+            sim.set_composite(pieces[:p])
+            sim.surrogate_init(seed=4)
+            sim.surrogate_make_data(samples, seed=4)
+            t0 = time.perf_counter()
+            sim.surrogate_train(steps=steps, batch=8192, lr=3e-3, lr_min=1e-5, seed=4)
+            t1 = time.perf_counter()
+            rmse = sim.surrogate_validate(400_000, seed=12345)
+            if steps == 3000:
+                res["pieces"][str(p)] = rmse
+            if p == 512:
+                res["training_steps_512"][str(steps)] = rmse
+            log(f"capacity: {p} pieces, {steps} steps ({t1 - t0:.2f} s): rel force RMSE {rmse:.4f}")
+    return res
+
+
 def bench_membrane(steps=200, frames=20):
     """The accuracy gates where the geometry is learnable: a membrane layer
     (one disk wider than a 400 x 400 x 40 nm box, 120,000 particles).
@@ -535,7 +564,19 @@ def cytosol_reactions(spec, boxes, counts, xyz, types, pieces, mtypes, w, norm, 
     return out
 
 
-def bench_cytosol(steps=100, warmup=10, rdf_steps=200, frames=10, plain_steps=20, train_steps=3000):
+def cytosol_rdf(sim, every):
+    """g(r) per replica over the configured 1,000 steps, a frame every `every` steps."""
+    from readdygpu_amd import workloads
+    steps = workloads.CYTOSOL["steps"]
+    for _ in range(steps // every):
+        sim.simulate(every)
+        sim.call_rdf_calculation()
+    g = sim.normalize_rdf()
+    sim.close()
+    return g
+
+
+def bench_cytosol(steps=100, warmup=10, rdf_every=2, plain_steps=20, train_steps=3000):
     """Combined workload: 64 replicas (5.06M particles, phi = 0.3, 8 species,
     two wells, A+B candidates) with a 256-disk membrane acting on the two
     largest species.  Batched + surrogate against batched exact (the same
@@ -579,22 +620,10 @@ def bench_cytosol(steps=100, warmup=10, rdf_steps=200, frames=10, plain_steps=20
                      "sort_avg_us": sort / steps * 1e3}
         out[mode]["candidates_ab"] = int(len(sim.reaction_candidates()))
         sim.close()
-        # RDF from the common start (fresh handle, same noise)
-        sim = handle(mode == "surrogate", w, norm)
-        sim.simulate(rdf_steps)
-        for _ in range(frames):
-            sim.simulate(10)
-            sim.call_rdf_calculation()
-        g[mode] = sim.normalize_rdf()
-        sim.close()
+        # RDF over the configured run from the common start (fresh handle, same noise)
+        g[mode] = cytosol_rdf(handle(mode == "surrogate", w, norm), rdf_every)
     # statistical floor: the exact run with another noise seed
-    sim = handle(False, sp=dataclasses.replace(spec, seed=spec.seed + 1000))
-    sim.simulate(rdf_steps)
-    for _ in range(frames):
-        sim.simulate(10)
-        sim.call_rdf_calculation()
-    g["exact_seed2"] = sim.normalize_rdf()
-    sim.close()
+    g["exact_seed2"] = cytosol_rdf(handle(False, sp=dataclasses.replace(spec, seed=spec.seed + 1000)), rdf_every)
 
     def rel(a, b):
         per = [float(np.linalg.norm(g[a][r] - g[b][r]) / max(np.linalg.norm(g[b][r]), 1e-30))
@@ -604,8 +633,12 @@ def bench_cytosol(steps=100, warmup=10, rdf_steps=200, frames=10, plain_steps=20
         return {"mean": float(np.mean(per)), "max": float(np.max(per)),
                 "pooled": float(np.linalg.norm(pa - pb) / np.linalg.norm(pb))}
     res.update(out)
+    steps_cfg = workloads.CYTOSOL["steps"]
+    res["rdf_sampling"] = (f"{steps_cfg} steps from the initial state, a frame every {rdf_every} steps "
+                           f"({steps_cfg // rdf_every} frames), 0-20 nm in 100 bins, relative L2 per replica")
     res["rdf_rel_l2_surrogate_vs_exact"] = rel("surrogate", "exact")
     res["rdf_rel_l2_exact_other_seed_vs_exact"] = rel("exact_seed2", "exact")
+    res["rdf_gate"] = 0.005
     # plain form: each replica on its own handle with the exact membrane
     tot = 0.0
     off = 0
@@ -642,7 +675,7 @@ def main():
     ap.add_argument("--ppt", type=int, default=0, help="force-kernel particles per lane (1, 2, 4; 0: autotuned)")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned workloads")
     ap.add_argument("--only", choices=["batched_replicas", "learned_slab", "learned_membrane", "crowded_cytosol",
-                                       "frame_handoff"],
+                                       "frame_handoff", "learned_capacity"],
                     default=None,
                     help="run only this extra workload and print its JSON object")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -657,7 +690,7 @@ def main():
     if args.only:
         fn = {"batched_replicas": bench_batched, "learned_slab": bench_learned,
               "learned_membrane": bench_membrane, "crowded_cytosol": bench_cytosol,
-              "frame_handoff": bench_frames}[args.only]
+              "frame_handoff": bench_frames, "learned_capacity": bench_capacity}[args.only]
         print(json.dumps(fn()))
         return
     spec, xyz, types = workloads.plain_1m(n=args.n, cell_size=args.cell)
