@@ -91,16 +91,18 @@ def host_cpu():
     return {"model": model, "nproc": os.cpu_count(), "usable": usable}
 
 
-def bench_handle(sim, steps, warmup, tune_steps=0):
+def bench_handle(sim, steps, warmup, tune_steps=0, direct=False):
     """(wall ms of the graph pass, force ms, sort ms, external-pass ms of the
     event pass) for `steps` steps after `warmup` steps; with tune_steps the
     autotuner runs on the warmed-up state (it restores the state, so the
-    trajectory does not change)."""
+    trajectory does not change).  direct: no graph capture (the wall time is
+    then that of the directly launched, event-timed steps)."""
     sim.simulate(warmup)
     if tune_steps:
         sim.autotune(trial_steps=tune_steps)
     out = (C.c_double * 5)()
-    rc = sim.lib.rd_bench_steps(sim.h, steps, out)
+    fn = sim.lib.rd_bench_steps_direct if direct else sim.lib.rd_bench_steps
+    rc = fn(sim.h, steps, out)
     if rc != 0:
         raise RuntimeError(sim.lib.rd_last_error(sim.h).decode())
     sim.simulate(0)
@@ -221,7 +223,7 @@ def rdf_l2(g_a, g_b):
     return float(np.linalg.norm(g_a - g_b) / max(np.linalg.norm(g_b), 1e-30))
 
 
-def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000):
+def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000, direct=False, save=None):
     """2M particles in a 2000x2000x400 nm slab with the 512-piece composite:
     exact external pass against the trained BF16 surrogate."""
     import numpy as np
@@ -247,9 +249,11 @@ def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000):
         f"rel force RMSE {rmse:.4f}")
     w = sim.surrogate_weights()
     lo, hi, gsc = sim.surrogate_norm()
+    if save:
+        np.savez(save, w=w, lo=lo, hi=hi, gscale=gsc)
     # exact external pass (this handle), then a fresh handle with the same
     # start, seed and step counter running the surrogate: identical noise
-    wall_e, force_e, sort_e, ext_e = bench_handle(sim, steps, warmup, tune_steps=3)
+    wall_e, force_e, sort_e, ext_e = bench_handle(sim, steps, warmup, tune_steps=3, direct=direct)
     sim.call_rdf_calculation()
     g_exact = sim.normalize_rdf()
     sim.close()
@@ -258,7 +262,7 @@ def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000):
     sim.set_surrogate_weights(w)
     sim.set_surrogate_norm(lo, hi, gsc)
     sim.set_external_mode(True)
-    wall_s, force_s, sort_s, ext_s = bench_handle(sim, steps, warmup, tune_steps=3)
+    wall_s, force_s, sort_s, ext_s = bench_handle(sim, steps, warmup, tune_steps=3, direct=direct)
     sim.call_rdf_calculation()
     g_sur = sim.normalize_rdf()
     sim.close()
@@ -280,6 +284,34 @@ def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000):
     log(f"learned: exact {outs['exact']['value']:.3e} p-steps/s (ext {outs['exact']['ext_avg_us']:.0f} us), "
         f"surrogate {outs['surrogate']['value']:.3e} (ext {outs['surrogate']['ext_avg_us']:.0f} us), "
         f"RDF rel L2 {res['rdf_rel_l2']:.4f}")
+    return res
+
+
+def learned_kernels(weights, steps=20):
+    """The learned slab's external passes and force kernel for kernel-trace
+    profiling: no training and no graph capture in this process (the trained
+    weights come from a file written by --only learned_slab --save-surrogate);
+    `steps` steps each with the exact pass and with the surrogate, launched
+    directly."""
+    import numpy as np
+    from readdygpu_amd import Simulation, workloads
+    spec, pieces, xyz, types = workloads.learned_slab()
+    z = np.load(weights)
+    res = {}
+    for name, sur in (("exact", False), ("surrogate", True)):
+        with Simulation(spec, xyz, types) as sim:
+            sim.set_composite(pieces)
+            if sur:
+                sim.set_surrogate_weights(z["w"])
+                sim.set_surrogate_norm(z["lo"], z["hi"], float(z["gscale"]))
+                sim.set_external_mode(True)
+            out = (C.c_double * 5)()
+            rc = sim.lib.rd_bench_steps_direct(sim.h, steps, out)
+            if rc != 0:
+                raise RuntimeError(sim.lib.rd_last_error(sim.h).decode())
+        res[name] = {"ext_avg_us": out[4] / steps * 1e3, "force_avg_us": out[2] / steps * 1e3,
+                     "sort_avg_us": out[3] / steps * 1e3}
+        log(f"learned kernels ({name}): ext {res[name]['ext_avg_us']:.1f} us, force {res[name]['force_avg_us']:.1f} us")
     return res
 
 
@@ -678,6 +710,13 @@ def main():
                                        "frame_handoff", "learned_capacity"],
                     default=None,
                     help="run only this extra workload and print its JSON object")
+    ap.add_argument("--direct", action="store_true",
+                    help="with --only learned_slab: time the steps without graph capture (for profilers)")
+    ap.add_argument("--save-surrogate", default=None, metavar="NPZ",
+                    help="with --only learned_slab: write the trained weights and normalisation")
+    ap.add_argument("--learned-kernels", default=None, metavar="NPZ",
+                    help="only launch the learned slab's kernels with the weights of NPZ, no training and no graph "
+                         "capture (kernel-trace profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES bytes (profiler HBM-counter calibration)")
     args = ap.parse_args()
@@ -687,11 +726,17 @@ def main():
 
     from readdygpu_amd import Simulation, workloads
 
+    if args.learned_kernels:
+        print(json.dumps(learned_kernels(args.learned_kernels)))
+        return
     if args.only:
         fn = {"batched_replicas": bench_batched, "learned_slab": bench_learned,
               "learned_membrane": bench_membrane, "crowded_cytosol": bench_cytosol,
               "frame_handoff": bench_frames, "learned_capacity": bench_capacity}[args.only]
-        print(json.dumps(fn()))
+        if args.only == "learned_slab":
+            print(json.dumps(fn(direct=args.direct, save=args.save_surrogate)))
+        else:
+            print(json.dumps(fn()))
         return
     spec, xyz, types = workloads.plain_1m(n=args.n, cell_size=args.cell)
     if args.calibrate_copy:

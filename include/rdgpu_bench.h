@@ -22,6 +22,12 @@ extern "C" {
  * surrogate; 0 without).  out holds 5 values. */
 int rd_bench_steps(rd_system* h, int nsteps, double* out);
 
+/* The same measurement without graph capture: nsteps steps launched directly
+ * with the events of pass 2 of rd_bench_steps; out[0] = host wall time of the
+ * launches + synchronize, out[1..4] as rd_bench_steps.  For profilers that
+ * should not go through graph capture and launch. */
+int rd_bench_steps_direct(rd_system* h, int nsteps, double* out);
+
 /* Copy `bytes` bytes device-to-device with a plain vector-copy kernel, reps
  * times, on the handle's stream (profiler calibration of HBM counters).
  * ms: average kernel time. */

@@ -182,6 +182,7 @@ _SIGS = {
     "rd_surrogate_predict": (C.c_int, [P, C.c_long, P, P]),
     "rd_surrogate_validate": (C.c_int, [P, C.c_long, C.c_uint64, C.POINTER(C.c_double)]),
     "rd_bench_steps": (C.c_int, [P, C.c_int, P]),
+    "rd_bench_steps_direct": (C.c_int, [P, C.c_int, P]),
     "rd_bench_copy": (C.c_int, [P, C.c_long, C.c_int, C.POINTER(C.c_double)]),
     "rd_bench_ext": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
 }

@@ -144,3 +144,47 @@ done:
     return rc;
 }
 
+extern "C" int rd_bench_steps_direct(rd_system* h, int nsteps, double* out)
+{
+    if (!h || nsteps < 1 || !out) return RD_EINVAL;
+    int rc = rd_host_refresh(h);
+    if (rc) return rc;
+    std::vector<hipEvent_t> ev(5 * (size_t)nsteps, nullptr);
+    double tf = 0.0, ts = 0.0, te = 0.0;
+    const bool ext = h->ext_active();
+    float ms = 0.0f;
+    std::chrono::steady_clock::time_point t0, t1;
+    for (auto& e : ev) BCK(hipEventCreate(&e));
+    BCK(hipStreamSynchronize(h->stream));
+    t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < nsteps; ++i) {
+        BCK(hipEventRecord(ev[5 * i], h->stream));
+        if (ext) rd_host_launch_ext(h);
+        BCK(hipEventRecord(ev[5 * i + 1], h->stream));
+        rd_host_launch_force(h, ev[5 * i + 2], ev[5 * i + 3]);
+        rd_host_launch_sort(h, 1);
+        BCK(hipEventRecord(ev[5 * i + 4], h->stream));
+    }
+    BCK(hipStreamSynchronize(h->stream));
+    t1 = std::chrono::steady_clock::now();
+    h->step += (uint64_t)nsteps;
+    for (int i = 0; i < nsteps; ++i) {
+        BCK(hipEventElapsedTime(&ms, ev[5 * i], ev[5 * i + 1]));
+        te += ms;
+        BCK(hipEventElapsedTime(&ms, ev[5 * i + 2], ev[5 * i + 3]));
+        tf += ms;
+        BCK(hipEventElapsedTime(&ms, ev[5 * i + 3], ev[5 * i + 4]));
+        ts += ms;
+    }
+    out[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    BCK(hipEventElapsedTime(&ms, ev[0], ev[5 * (size_t)nsteps - 1]));
+    out[1] = ms;
+    out[2] = tf;
+    out[3] = ts;
+    out[4] = te;
+done:
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
+}
+
