@@ -18,7 +18,10 @@ crowded_cytosol (64 replicas, 5.06M particles, 256-disk membrane on species 6-7)
     for the three smallest replicas;
   * one frame of A+B -> C through rd_apply_reactions_replicas at full size:
     per-replica and per-type counts, positions inside their boxes, the sorted
-    state's invariants, and a step after it.
+    state's invariants, and a step after it;
+  * the surrogate's gain at the accuracy gates: force RMSE <= 1%, pooled RDF
+    within 0.5% of the exact run, a faster external pass and >= 3x the plain
+    form's particle-steps/s.
 """
 import numpy as np
 import pytest
@@ -136,3 +139,79 @@ def test_crowded_cytosol_full_size():
         sim.simulate(3)
         x3, _ = sim.copy_pos_from_device()
         assert np.isfinite(x3).all() and sim.check_invariants() == 0
+
+
+def _ext_pass_ms(sim, reps=10):
+    import ctypes as C
+    ms = C.c_double()
+    assert sim.lib.rd_bench_ext(sim.h, reps, C.byref(ms)) == 0
+    return ms.value
+
+
+def _steps_per_s(sim, n, steps=40):
+    import time
+    sim.simulate(steps)                        # graphs captured, state warmed
+    t0 = time.perf_counter()
+    sim.simulate(steps)
+    return n * steps / (time.perf_counter() - t0)
+
+
+def test_cytosol_surrogate_gain_at_the_gates():
+    """What the trained network gains on the crowded cytosol: at force RMSE
+    <= 1% on fresh samples and an RDF pooled over the replicas within 0.5%
+    (relative L2) of the exact run from the same start with the same noise
+    (200 steps, a frame every 2 steps), the surrogate's external pass is
+    faster than the exact pass, and the batched step with it delivers >= 3x
+    the particle-steps/s of the plain form (each replica on its own handle
+    with the exact membrane)."""
+    import dataclasses
+    spec, boxes, counts, xyz, types, pieces, mtypes = workloads.crowded_cytosol()
+    n = int(counts.sum())
+    lmax = float(boxes[:, 1].max())
+
+    def handle(w=None, norm=None):
+        sim = Simulation(spec, xyz, types, replicas=(boxes, counts), rdf=(20.0, 100))
+        sim.set_composite(pieces, types=mtypes)
+        if w is not None:
+            sim.set_surrogate_weights(w)
+            sim.set_surrogate_norm(*norm)
+            sim.set_external_mode(True)
+        return sim
+
+    def pooled_rdf(sim):
+        for _ in range(100):
+            sim.simulate(2)
+            sim.call_rdf_calculation()
+        g = sim.normalize_rdf()
+        return (g * counts[:, None, None, None]).sum(0)
+
+    with handle() as ex:
+        ex.surrogate_init(seed=5)
+        ex.set_surrogate_norm((0.0, 0.0, 0.0), (lmax, lmax, lmax), 1.0)
+        ex.surrogate_make_data(1 << 22, seed=5)
+        ex.surrogate_train(steps=3000, batch=8192, lr=3e-3, lr_min=1e-5, seed=5)
+        rmse = ex.surrogate_validate(400_000, seed=55)
+        w, norm = ex.surrogate_weights(), ex.surrogate_norm()
+        ms_exact = _ext_pass_ms(ex)
+        g_exact = pooled_rdf(ex)
+    with handle(w, norm) as su:
+        ms_sur = _ext_pass_ms(su)
+        g_sur = pooled_rdf(su)
+        batched = _steps_per_s(su, n)
+    dev = float(np.linalg.norm(g_sur - g_exact) / np.linalg.norm(g_exact))
+    t_plain = 0.0
+    off = 0
+    for r, c in enumerate(counts):
+        b = boxes[r]
+        s1 = dataclasses.replace(spec, box_lo=(b[0], b[2], b[4]), box_hi=(b[1], b[3], b[5]), stream=r)
+        with Simulation(s1, xyz[off:off + c], types[off:off + c]) as one:
+            one.set_composite(pieces, types=mtypes)
+            t_plain += int(c) * 40 / _steps_per_s(one, int(c))
+        off += c
+    plain = n * 40 / t_plain
+    print(f"cytosol: force rel RMSE {rmse:.4f}, pooled RDF rel L2 {dev:.5f}, external pass exact {ms_exact:.3f} ms "
+          f"surrogate {ms_sur:.3f} ms, batched+surrogate {batched:.3e} plain {plain:.3e} particle-steps/s")
+    assert rmse <= 0.01
+    assert dev <= 0.005
+    assert ms_sur < ms_exact
+    assert batched >= 3.0 * plain
