@@ -141,13 +141,13 @@ def fp64_error(steps=20):
 
 def configured_run(spec, xyz, types, lc):
     """The 1M workload as configured (500 steps from the initial state) with
-    the tuned cell size and launch shape: host wall time of the 500 steps as
+    the tuned cell size and force-kernel form and shape: host wall time of the 500 steps as
     one graph (rd_bench_steps pass 1), and the force-kernel average over the
     500 steps after them."""
     from readdygpu_amd import Simulation, workloads
     steps = workloads.PLAIN_1M["steps"]
     with Simulation(spec, xyz, types) as sim:
-        sim.set_launch_config(kernel=0, block=lc["used_block"], ppt=lc["used_ppt"])
+        sim.set_launch_config(kernel=lc["used_kernel"], block=lc["used_block"], ppt=lc["used_ppt"])
         out = (C.c_double * 5)()
         rc = sim.lib.rd_bench_steps(sim.h, steps, out)
         if rc != 0:

@@ -42,6 +42,27 @@ def _hessian(pieces, x, h=1e-3):
     return 0.5 * (H + H.T)
 
 
+def test_composite_sum_is_the_sum_of_the_pieces():
+    """orc_composite_sum against the pieces one at a time (each piece's
+    energy is pinned to 0.5 k dist^2 by brute-force distances in
+    test_oracle_surrogate.py) and its gradient against central differences
+    of its own energy."""
+    spec, pieces, xyz, types = _slab(64)
+    sub = pieces[::16]
+    x = xyz[:12].astype(np.float64)
+    g, e = orc.composite_sum(sub, x)
+    for i in range(len(x)):
+        parts = [orc.piece_eval(p, x[i]) for p in sub]
+        assert abs(e[i] - sum(pe for pe, _ in parts)) <= 1e-9 * max(1.0, abs(e[i]))
+        assert np.allclose(g[i], np.sum([pg for _, pg in parts], axis=0), rtol=1e-12, atol=1e-9)
+        h = 1e-4
+        for d in range(3):
+            xp = x[i].copy(); xp[d] += h
+            xm = x[i].copy(); xm[d] -= h
+            fd = (orc.composite_sum(sub, xp[None])[1][0] - orc.composite_sum(sub, xm[None])[1][0]) / (2 * h)
+            assert abs(fd - g[i, d]) <= 1e-5 * max(1.0, np.abs(g[i]).max()), (i, d)
+
+
 def test_summed_slab_curvature_exceeds_the_euler_maruyama_bound():
     spec, pieces, xyz, types = _slab(64)
     k = pieces[0].k
