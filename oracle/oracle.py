@@ -77,6 +77,7 @@ def lib():
             "orc_composite_sum": (None, [V, C.c_int, C.c_long, V, V, V, C.c_int]),
             "orc_bf16": (C.c_float, [C.c_float]),
             "orc_mlp_forward": (None, [V, C.c_long, V, C.c_int, V]),
+            "orc_mlp_forward_infer": (None, [V, C.c_long, V, C.c_int, V]),
             "orc_mlp_loss_grad": (C.c_double, [V, C.c_long, V, V, C.c_int, V]),
         }
         for k, (r, a) in sig.items():
@@ -308,11 +309,14 @@ def bf16(v):
     return np.where(special, a, out)
 
 
-def mlp_forward(w, xn, bf16_round=False):
+def mlp_forward(w, xn, bf16_round=False, inference=False):
+    """The MLP at FP64; bf16_round: BF16 at the kernels' rounding points;
+    inference: in the inference kernels' scaled form (orc_mlp_forward_infer)."""
     w = np.ascontiguousarray(w, dtype=np.float64)
     x = np.ascontiguousarray(xn, dtype=np.float32).reshape(-1, 3)
     y = np.zeros((len(x), 3))
-    lib().orc_mlp_forward(_p(w), len(x), _p(x), int(bf16_round), _p(y))
+    fn = lib().orc_mlp_forward_infer if inference else lib().orc_mlp_forward
+    fn(_p(w), len(x), _p(x), int(bf16_round), _p(y))
     return y
 
 

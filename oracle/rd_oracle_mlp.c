@@ -156,6 +156,53 @@ void orc_mlp_forward(const double* w, long m, const float* xn, int bf16, double*
     free(M);
 }
 
+/* inference form (rd_surrogate.hip: mlp_tile, pack_param's w1s/b1s/b2s/b3s/w4s) */
+void orc_mlp_forward_infer(const double* w, long m, const float* xn, int bf16, double* y)
+{
+    const double nl2e = -1.0 / log(2.0), nln2 = -log(2.0);
+    mlp_w* M = (mlp_w*)malloc(sizeof(mlp_w));
+    load_w(w, bf16, M);                 /* W2, W3 as the kernels hold them */
+    for (int i = 0; i < 64; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            const double v = w[ORC_MLP_W1 + 3 * i + c];
+            M->W1[i][c] = bf16 ? (double)((float)orc_bf16((float)v) * -1.44269504f) : v * nl2e;
+        }
+        M->b1[i] = bf16 ? (double)((float)w[ORC_MLP_B1 + i] * -1.44269504f) : w[ORC_MLP_B1 + i] * nl2e;
+        M->b2[i] = bf16 ? (double)((float)w[ORC_MLP_B2 + i] * -1.44269504f) : w[ORC_MLP_B2 + i] * nl2e;
+        M->b3[i] = bf16 ? (double)((float)w[ORC_MLP_B3 + i] * -1.44269504f) : w[ORC_MLP_B3 + i] * nl2e;
+    }
+    for (int o = 0; o < 3; ++o)
+        for (int k = 0; k < 64; ++k) {
+            const double v = w[ORC_MLP_W4 + 64 * o + k];
+            M->W4[o][k] = bf16 ? (double)orc_bf16((float)v * -0.693147181f) : v * nln2;
+        }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (long s = 0; s < m; ++s) {
+        double h[64], hn[64];
+        for (int i = 0; i < 64; ++i) {
+            double z = M->b1[i];
+            for (int c = 0; c < 3; ++c) z += M->W1[i][c] * (double)xn[3 * s + c];
+            h[i] = rnd(z / (1.0 + exp2(z)), bf16);
+        }
+        for (int l = 0; l < 2; ++l) {
+            for (int i = 0; i < 64; ++i) {
+                double z = l == 0 ? M->b2[i] : M->b3[i];
+                for (int k = 0; k < 64; ++k) z += (l == 0 ? M->W2[i][k] : M->W3[i][k]) * h[k];
+                hn[i] = rnd(z / (1.0 + exp2(z)), bf16);
+            }
+            memcpy(h, hn, sizeof(h));
+        }
+        for (int o = 0; o < 3; ++o) {
+            double v = M->b4[o];
+            for (int k = 0; k < 64; ++k) v += M->W4[o][k] * h[k];
+            y[3 * s + o] = v;
+        }
+    }
+    free(M);
+}
+
 static double dsilu(double z)
 {
     double s = sigm(z);

@@ -53,6 +53,13 @@ float orc_bf16(float v);
 
 /* forward pass: y[m][3] */
 void orc_mlp_forward(const double* w, long m, const float* xn, int bf16, double* y);
+/* The same network in the inference form of the kernels: layer 1 and the
+ * biases times -log2(e), W4 times -ln 2, activations zs / (1 + 2^zs) =
+ * -log2(e) SiLU(z).  bf16 = 0: exact scale factors in FP64 (the network
+ * itself, to rounding); bf16 != 0: the kernels' FP32 products and BF16
+ * roundings (W1 rounded to BF16 before its FP32 scaling, W4's product
+ * rounded to BF16, activations rounded to BF16). */
+void orc_mlp_forward_infer(const double* w, long m, const float* xn, int bf16, double* y);
 
 /* loss = (1/m) sum_i |y_i - t_i|^2 and its gradient with respect to w */
 double orc_mlp_loss_grad(const double* w, long m, const float* xn, const float* t, int bf16, double* grad);

@@ -6,6 +6,10 @@ Flags:
                                          RDF bins, noise)
   -fhip-fp32-correctly-rounded-divide-sqrt  IEEE division and sqrt
   -Xclang -target-feature -Xclang -packed-fp32-ops   no packed FP32 VALU ops
+  rd_surrogate.hip also: -mllvm -amdgpu-mfma-vgpr-form=1
+                                         MFMA results in VGPRs (the SiLU that
+                                         follows each MFMA reads them without
+                                         accumulator-register moves)
 """
 from __future__ import annotations
 
@@ -29,6 +33,9 @@ FLAGS = [
 ]
 
 
+EXTRA = {"rd_surrogate.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
@@ -41,7 +48,7 @@ def build(verbose: bool = False) -> str:
     for src in sources():
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + FLAGS + EXTRA.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
         if src.endswith(".cpp"):
             cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", src, "-o", obj]
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

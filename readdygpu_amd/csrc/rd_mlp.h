@@ -41,7 +41,15 @@ struct RdMlpPack {
     rd_bf16x8 w4t[4][64];        // A = W4^T, K = 3 real of 32
     float w1[64 * 3];            // BF16-rounded W1 as FP32
     float b1[64], b2[64], b3[64], b4[4];
+    // inference copies: layer 1 and the biases times -log2(e), W4 times
+    // -ln 2 (the forward pass then works on -log2(e) z throughout)
+    float w1s[64 * 3];
+    float b1s[64], b2s[64], b3s[64];
+    rd_bf16x8 w4s[2][64];
 };
+
+#define RD_MLP_NLOG2E -1.44269504f   // -log2(e)
+#define RD_MLP_NLN2 -0.693147181f    // -ln(2) = 1 / -log2(e)
 
 // position normalisation and output scale of a trained surrogate
 struct RdMlpNorm {
@@ -180,75 +188,4 @@ __device__ __forceinline__ float rd_sum16(float v)
     v += __shfl_xor(v, 4, 64);
     v += __shfl_xor(v, 8, 64);
     return v;
-}
-
-// ---- inference forms: activations kept only as packed BF16 B operands ----
-// hb[tj][kb] = B operand of sample tile tj, K-block kb (the SiLU output
-// rounded to BF16, exactly what the next MFMA consumes)
-__device__ __forceinline__ void rd_mlp_pack_silu(const rd_f32x4 (&z)[4], int tj, rd_bf16x8 (&hb)[4][2])
-{
-    // z[ti] holds features 16 ti + 4 g + r of sample tile tj
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-        for (int t = 0; t < 8; ++t) hb[tj][kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
-}
-
-__device__ __forceinline__ void rd_mlp_layer1_b(const float* __restrict__ w1, const float* __restrict__ b1,
-                                                const float (&xn)[4][3], rd_bf16x8 (&hb)[4][2], int lane)
-{
-    const int g = lane >> 4;
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
-        rd_f32x4 z[4];
-#pragma unroll
-        for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int f = ti * 16 + 4 * g + r;
-                float v = w1[3 * f] * xn[tj][0];
-                v = v + w1[3 * f + 1] * xn[tj][1];
-                v = v + w1[3 * f + 2] * xn[tj][2];
-                z[ti][r] = v + b1[f];
-            }
-        rd_mlp_pack_silu(z, tj, hb);
-    }
-}
-
-template <class AF>
-__device__ __forceinline__ void rd_mlp_layer64_b(AF&& A, const float* __restrict__ b, const rd_bf16x8 (&hin)[4][2],
-                                                 rd_bf16x8 (&hout)[4][2], int lane)
-{
-    const int g = lane >> 4;
-    rd_f32x4 bias[4];
-#pragma unroll
-    for (int to = 0; to < 4; ++to)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) bias[to][r] = b[to * 16 + 4 * g + r];
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
-        rd_f32x4 z[4];
-#pragma unroll
-        for (int to = 0; to < 4; ++to) {
-            z[to] = RD_MFMA(A(to, 0), hin[tj][0], bias[to]);
-            z[to] = RD_MFMA(A(to, 1), hin[tj][1], z[to]);
-        }
-        rd_mlp_pack_silu(z, tj, hout);
-    }
-}
-
-template <class AF>
-__device__ __forceinline__ void rd_mlp_layer4_b(AF&& A, const float* __restrict__ b4, const rd_bf16x8 (&hb)[4][2],
-                                                float (&y)[4][3], int lane)
-{
-    const int g = lane >> 4;
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
-        rd_f32x4 acc;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = (g == 0 && r < 3) ? b4[r] : 0.0f;
-        acc = RD_MFMA(A(0), hb[tj][0], acc);
-        acc = RD_MFMA(A(1), hb[tj][1], acc);
-        y[tj][0] = acc[0]; y[tj][1] = acc[1]; y[tj][2] = acc[2];
-    }
 }

@@ -512,8 +512,16 @@ __device__ __forceinline__ void pack_param(RdMlpPack* __restrict__ pk, int p, fl
         t = 4 * ((b >> 4) & 1) + (b & 3);
         g = (b & 15) >> 2;
     };
-    if (p < RD_MLP_B1) { pk->w1[p] = (float)wb; return; }
-    if (p < RD_MLP_W2) { pk->b1[p - RD_MLP_B1] = w; return; }
+    if (p < RD_MLP_B1) {
+        pk->w1[p] = (float)wb;
+        pk->w1s[p] = (float)wb * RD_MLP_NLOG2E;
+        return;
+    }
+    if (p < RD_MLP_W2) {
+        pk->b1[p - RD_MLP_B1] = w;
+        pk->b1s[p - RD_MLP_B1] = w * RD_MLP_NLOG2E;
+        return;
+    }
     if (p < RD_MLP_B2 || (p >= RD_MLP_W3 && p < RD_MLP_B3)) {
         const bool l3 = p >= RD_MLP_W3;
         const int q = p - (l3 ? RD_MLP_W3 : RD_MLP_W2);
@@ -525,14 +533,23 @@ __device__ __forceinline__ void pack_param(RdMlpPack* __restrict__ pk, int p, fl
         put(l3 ? &pk->w3t[b >> 4][kb][g * 16 + (b & 15)] : &pk->w2t[b >> 4][kb][g * 16 + (b & 15)], t);
         return;
     }
-    if (p < RD_MLP_W3) { pk->b2[p - RD_MLP_B2] = w; return; }
-    if (p < RD_MLP_W4) { pk->b3[p - RD_MLP_B3] = w; return; }
+    if (p < RD_MLP_W3) {
+        pk->b2[p - RD_MLP_B2] = w;
+        pk->b2s[p - RD_MLP_B2] = w * RD_MLP_NLOG2E;
+        return;
+    }
+    if (p < RD_MLP_W4) {
+        pk->b3[p - RD_MLP_B3] = w;
+        pk->b3s[p - RD_MLP_B3] = w * RD_MLP_NLOG2E;
+        return;
+    }
     if (p < RD_MLP_B4) {
         const int q = p - RD_MLP_W4, o = q >> 6, b = q & 63;
         int kb, g, t;
         kpos(b, kb, g, t);
         put(&pk->w4[kb][g * 16 + o], t);
         put(&pk->w4t[b >> 4][b & 15], o);
+        reinterpret_cast<__bf16*>(&pk->w4s[kb][g * 16 + o])[t] = (__bf16)(w * RD_MLP_NLN2);
         return;
     }
     pk->b4[p - RD_MLP_B4] = w;
@@ -598,11 +615,21 @@ __global__ void k_mlp_step_inc(RdTrainCtl* ctl) { ctl->step += 1; }
 // ===========================================================================
 // Persistent waves: the forward weights (A fragments, 18 KB) and layer-1
 // weights and biases are staged in LDS per workgroup; the compiler keeps the
-// fragments a lane uses in registers across tiles (about 180 VGPRs: 2 waves
+// fragments a lane uses in registers across tiles (about 210 VGPRs: 2 waves
 // per SIMD, which overlap one wave's SiLU work on the VALU with the other's
-// MFMAs).  Samples go through the network one sample tile (16 samples) at a
+// MFMAs; the MFMAs write VGPRs directly, built with
+// -mllvm -amdgpu-mfma-vgpr-form, so no accumulator-register reads follow
+// them).  Samples go through the network one sample tile (16 samples) at a
 // time, so only one tile's activations are live, as packed BF16 B operands.
 // Lane l holds sample l % 16 of the tile; outputs land in lanes 0..15.
+//
+// The staged copies are the pack's inference copies: layer 1 and the biases
+// carry -log2(e) and W4 carries -ln 2, so every layer yields zs = -log2(e) z
+// and the activation zs / (1 + 2^zs) = -log2(e) SiLU(z) feeds the next layer
+// unchanged (W2 and W3 see the factor in both their input and their output);
+// W4's -ln 2 removes it.  SiLU then costs 2^x, a reciprocal, an add and one
+// multiply.  The inference is VALU-bound (two transcendentals per
+// activation), so each VALU instruction saved counts.
 struct RdMlpFwdLds {
     rd_bf16x8 w2[4][2][64], w3[4][2][64], w4[2][64];
     float w1[192], b1[64], b2[64], b3[64], b4[4];
@@ -610,13 +637,26 @@ struct RdMlpFwdLds {
 
 __device__ __forceinline__ void mlp_stage(RdMlpFwdLds& L, const RdMlpPack* __restrict__ pk_g)
 {
-    const uint4* src = reinterpret_cast<const uint4*>(pk_g);      // w2, w3, w4 lead the pack
+    const uint4* src = reinterpret_cast<const uint4*>(pk_g);      // w2, w3 lead the pack
     uint4* dst = reinterpret_cast<uint4*>(&L);
-    const int n16 = (int)(sizeof(rd_bf16x8) * (2 * 4 * 2 * 64 + 2 * 64) / 16);
-    for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
-    for (int i = threadIdx.x; i < 192; i += blockDim.x) L.w1[i] = pk_g->w1[i];
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) { L.b1[i] = pk_g->b1[i]; L.b2[i] = pk_g->b2[i]; L.b3[i] = pk_g->b3[i]; }
+    const int n23 = (int)(sizeof(rd_bf16x8) * (2 * 4 * 2 * 64) / 16);
+    for (int i = threadIdx.x; i < n23; i += blockDim.x) dst[i] = src[i];
+    const uint4* s4 = reinterpret_cast<const uint4*>(pk_g->w4s);
+    uint4* d4 = reinterpret_cast<uint4*>(L.w4);
+    for (int i = threadIdx.x; i < 2 * 64; i += blockDim.x) d4[i] = s4[i];
+    for (int i = threadIdx.x; i < 192; i += blockDim.x) L.w1[i] = pk_g->w1s[i];
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+        L.b1[i] = pk_g->b1s[i];
+        L.b2[i] = pk_g->b2s[i];
+        L.b3[i] = pk_g->b3s[i];
+    }
     if (threadIdx.x < 4) L.b4[threadIdx.x] = pk_g->b4[threadIdx.x];
+}
+
+// -log2(e) SiLU(z) from zs = -log2(e) z
+__device__ __forceinline__ float rd_silu_l2(float zs)
+{
+    return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs));
 }
 
 // network output for this lane's sample of the tile (valid in lanes 0..15)
@@ -624,23 +664,22 @@ __device__ __forceinline__ float3 mlp_tile(const RdMlpFwdLds& L, const float (&x
 {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4;
-    // layer 1 (FP32, VALU) -> SiLU -> B operands
+    // layer 1 (FP32 FMAs, VALU) -> SiLU -> B operands
     rd_f32x4 z[4];
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int f = ti * 16 + 4 * g + r;
-            float v = L.w1[3 * f] * xn[0];
-            v = v + L.w1[3 * f + 1] * xn[1];
-            v = v + L.w1[3 * f + 2] * xn[2];
-            z[ti][r] = v + L.b1[f];
+            float v = __builtin_fmaf(L.w1[3 * f], xn[0], L.b1[f]);
+            v = __builtin_fmaf(L.w1[3 * f + 1], xn[1], v);
+            z[ti][r] = __builtin_fmaf(L.w1[3 * f + 2], xn[2], v);
         }
     rd_bf16x8 hb[2];
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-        for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
+        for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu_l2(z[2 * kb + t / 4][t & 3]);
     // layers 2 and 3 (MFMA) -> SiLU
 #pragma unroll
     for (int layer = 0; layer < 2; ++layer) {
@@ -657,7 +696,7 @@ __device__ __forceinline__ float3 mlp_tile(const RdMlpFwdLds& L, const float (&x
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-            for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu(z[2 * kb + t / 4][t & 3]);
+            for (int t = 0; t < 8; ++t) hb[kb][t] = (__bf16)rd_silu_l2(z[2 * kb + t / 4][t & 3]);
     }
     // layer 4 (MFMA, rows 0..2 of 16)
     rd_f32x4 acc;

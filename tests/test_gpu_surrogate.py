@@ -79,7 +79,8 @@ def test_mlp_loss_gradient_matches_oracle():
         y = sim.surrogate_predict(xn)
     l16, g16 = orc.mlp_loss_grad(w, xn, t, True)
     l64, g64 = orc.mlp_loss_grad(w, xn, t, False)
-    y16 = orc.mlp_forward(w, xn, True)
+    y16 = orc.mlp_forward(w, xn, True, inference=True)     # the inference kernels' form
+    y64 = orc.mlp_forward(w, xn)
     assert abs(loss - l16) <= 2e-3 * abs(l16)
     r16 = np.linalg.norm(g - g16) / np.linalg.norm(g16)
     r64 = np.linalg.norm(g - g64) / np.linalg.norm(g64)
@@ -92,6 +93,7 @@ def test_mlp_loss_gradient_matches_oracle():
         blk = np.linalg.norm(g[a:b] - g16[a:b]) / max(np.linalg.norm(g16[a:b]), 1e-12)
         assert blk <= 1e-2, (a, b, blk)
     assert np.abs(y - y16).max() <= 5e-3 * np.abs(y16).max()
+    assert np.linalg.norm(y - y64) <= 1e-2 * np.linalg.norm(y64)
 
 
 def test_training_reduces_the_force_error():

@@ -65,6 +65,24 @@ def test_mlp_forward_matches_torch(bf16):
     assert np.abs(y - yt).max() <= tol
 
 
+def test_mlp_inference_form_is_the_network():
+    """The inference kernels' form (layer 1 and the biases times -log2 e, W4
+    times -ln 2, activations zs / (1 + 2^zs)) against torch's network: equal
+    to rounding without BF16, and within BF16 noise of the FP64 network with
+    the kernels' BF16 and FP32 roundings, as close as the plain BF16 form."""
+    rng = np.random.default_rng(11)
+    w = rand_weights(rng)
+    xn = rng.uniform(-1, 1, (300, 3)).astype(np.float32)
+    yt = torch_mlp(w, xn).numpy()
+    yi = orc.mlp_forward(w, xn, False, inference=True)
+    assert np.abs(yi - yt).max() <= 1e-12 * np.abs(yt).max()
+    yi16 = orc.mlp_forward(w, xn, True, inference=True)
+    y16 = orc.mlp_forward(w, xn, True)
+    e_inf = np.linalg.norm(yi16 - yt) / np.linalg.norm(yt)
+    e_bf = np.linalg.norm(y16 - yt) / np.linalg.norm(yt)
+    assert 1e-5 < e_inf <= 1e-2 and e_inf <= 3 * e_bf, (e_inf, e_bf)
+
+
 def test_mlp_gradient_matches_autograd_and_finite_differences():
     rng = np.random.default_rng(2)
     w = rand_weights(rng)
