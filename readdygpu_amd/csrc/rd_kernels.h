@@ -330,35 +330,78 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
             // Phase 2 evaluates the listed pairs only, so the interaction
             // code runs (max hits over the wave) times, not once per
             // candidate iteration in which some lane has a hit.
+            // The 27-cell stencil is 9 rows of at most two runs of
+            // contiguous slots (two only for homes at the x edge of the
+            // lattice): runs 0..8 are the first runs of rows 0..8, runs
+            // 9..17 the second runs.  Runs are walked in groups of three,
+            // whose six bounds are loaded together (independent loads in
+            // flight at once).
+            // Dense stencils: a lane whose list fills notes its first
+            // unlisted hit (run, slot) and stops listing; before its next
+            // run the wave leaves the listing-only walk for a second one
+            // that goes on from there, listing while a lane's list has room
+            // and evaluating further hits as they are met.  Keeping the
+            // interaction code out of the listing-only loops keeps their
+            // registers free (inlined there, it made the compiler reload the
+            // particle array's address from the kernel arguments on every
+            // iteration).
             int nhit = 0;
-            auto test = [&](uint32_t j, float4 q) {
-                float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            bool over = false;              // list full: (res_run, res_j) is the first unlisted hit
+            int cur_run = 0, res_run = 0;
+            uint32_t res_j = 0u;
+            auto delta = [&](float4 q, float& dx, float& dy, float& dz) {
+                dx = p.x - q.x; dy = p.y - q.y; dz = p.z - q.z;
                 if (!in) {
                     dx = min_image(dx, R.Lb[0], R.half[0]);
                     dy = min_image(dy, R.Lb[1], R.half[1]);
                     dz = min_image(dz, R.Lb[2], R.half[2]);
                 }
+            };
+            auto test = [&](uint32_t j, float4 q) {           // listing only
+                float dx, dy, dz;
+                delta(q, dx, dy, dz);
                 const float d2 = dist2(dx, dy, dz);
                 if (d2 < T.rc2[rowbase + w_type(q.w)] && d2 > 0.0f) {
-                    if (nhit < RD_HITS) il->j[nhit][lane] = j;
-                    else pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), acc);
-                    ++nhit;
+                    if (nhit < RD_HITS) {
+                        il->j[nhit][lane] = j;
+                        ++nhit;
+                    } else if (!over) {
+                        over = true;
+                        res_run = cur_run;
+                        res_j = j;
+                    }
                 }
             };
-            // 27-cell stencil as 9 rows of at most two runs of contiguous
-            // slots (two only for homes at the x edge of the lattice).  The
-            // first runs go in groups of three rows: the group's 6 bounds are
-            // loaded together (independent loads, in flight at once), then
-            // its runs are walked.  The second runs follow only in waves that
-            // have any.  Few bounds live at a time keep the kernel at 57
-            // VGPRs, 8 waves per SIMD, without spills.
+            auto test_dense = [&](uint32_t j, float4 q) {     // listing, then evaluating
+                float dx, dy, dz;
+                delta(q, dx, dy, dz);
+                const float d2 = dist2(dx, dy, dz);
+                if (d2 < T.rc2[rowbase + w_type(q.w)] && d2 > 0.0f) {
+                    if (nhit < RD_HITS) {
+                        il->j[nhit][lane] = j;
+                        ++nhit;
+                    } else {
+                        pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), acc);
+                    }
+                }
+            };
             int a0, b0, a1, b1;
             const int nr = x_runs(cx, 1, Lx, &a0, &b0, &a1, &b1);
+            const bool two = __any(nr == 2);
+            const int nruns = two ? 18 : 9;
             auto row_of = [&](int r) {
                 int yy = cy + (r % 3) - 1, zz = cz + (r / 3) - 1;
                 yy += yy < 0 ? Ly : 0; yy -= yy >= Ly ? Ly : 0;
                 zz += zz < 0 ? Lz : 0; zz -= zz >= Lz ? Lz : 0;
                 return R.coff + (zz * Ly + yy) * Lx;
+            };
+            auto bounds = [&](int run, uint32_t& j0, uint32_t& j1) {
+                j0 = 0u; j1 = 0u;
+                if (run < 9 || nr == 2) {
+                    const int row = row_of(run < 9 ? run : run - 9);
+                    j0 = start[row + (run < 9 ? a0 : a1)];
+                    j1 = start[row + (run < 9 ? b0 : b1) + 1];
+                }
             };
             auto walk = [&](uint32_t j, uint32_t j1) {
                 for (; j + 1 < j1; j += 2) {
@@ -368,40 +411,64 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                 }
                 if (j < j1) test(j, ps[j]);
             };
+            int rnext = nruns;              // first run the listing-only walk did not reach
+            bool stop = false;
 #pragma unroll
-            for (int g = 0; g < 9; g += 3) {
-                uint32_t sb[3], se[3];
+            for (int g = 0; g < 18; g += 3) {
+                if (g < nruns && !stop) {
+                    uint32_t sb[3], se[3];
 #pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const int row = row_of(g + r);
-                    sb[r] = start[row + a0];
-                    se[r] = start[row + b0 + 1];
+                    for (int r = 0; r < 3; ++r) bounds(g + r, sb[r], se[r]);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        if (!stop) {
+                            if (__any(over)) {
+                                stop = true;
+                                rnext = g + k;
+                            } else {
+                                cur_run = g + k;
+                                walk(sb[k], se[k]);
+                            }
+                        }
+                    }
                 }
-#pragma unroll
-                for (int k = 0; k < 3; ++k) walk(sb[k], se[k]);
             }
-            if (__any(nr == 2)) {
-                uint32_t sb[9], se[9];
+            if (__any(over)) {
+                // go on from the run in which lists filled (rnext - 1, the
+                // last one walked): lanes whose list filled from their first
+                // unlisted hit, the others from run rnext
+#pragma unroll 1
+                for (int r0 = rnext - 1; r0 < nruns; r0 += 3) {
+                    uint32_t sb[3], se[3];
 #pragma unroll
-                for (int r = 0; r < 9; ++r) {
-                    const int row = row_of(r);
-                    sb[r] = nr == 2 ? start[row + a1] : 0u;
-                    se[r] = nr == 2 ? start[row + b1 + 1] : 0u;
+                    for (int r = 0; r < 3; ++r) {
+                        const int run = r0 + r;
+                        sb[r] = 0u;
+                        se[r] = 0u;
+                        if (run < nruns && run >= (over ? res_run : rnext)) {
+                            bounds(run, sb[r], se[r]);
+                            if (over && run == res_run) sb[r] = res_j;
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        uint32_t j = sb[k];
+                        const uint32_t j1 = se[k];
+                        for (; j + 1 < j1; j += 2) {
+                            const float4 q0 = ps[j], q1 = ps[j + 1];
+                            test_dense(j, q0);
+                            test_dense(j + 1, q1);
+                        }
+                        if (j < j1) test_dense(j, ps[j]);
+                    }
                 }
-#pragma unroll
-                for (int k = 0; k < 9; ++k) walk(sb[k], se[k]);
             }
             // Phase 2
-            const int nl = nhit < RD_HITS ? nhit : RD_HITS;
-            for (int k = 0; __any(k < nl); ++k) {
-                if (k < nl) {
+            for (int k = 0; __any(k < nhit); ++k) {
+                if (k < nhit) {
                     const float4 q = ps[il->j[k][lane]];
-                    float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                    if (!in) {
-                        dx = min_image(dx, R.Lb[0], R.half[0]);
-                        dy = min_image(dy, R.Lb[1], R.half[1]);
-                        dz = min_image(dz, R.Lb[2], R.half[2]);
-                    }
+                    float dx, dy, dz;
+                    delta(q, dx, dy, dz);
                     pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), acc);
                 }
             }

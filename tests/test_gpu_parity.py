@@ -90,9 +90,17 @@ def test_cell_ids_bitexact(c1):
         assert np.array_equal(sim.cell_ids(), o.cell_index(x)[0])
 
 
-@pytest.mark.parametrize("which", ["plain_10k", "rich", "dense"])
+@pytest.mark.parametrize("which", ["plain_10k", "rich", "dense", "denser"])
 def test_gradients_and_energy_match_fp64(which, c1):
-    spec, xyz, types = c1 if which == "plain_10k" else (rich_spec() if which == "rich" else dense_spec())
+    """dense: 36 partners within range per particle on average (at most 68),
+    denser: 84 (at most 127), so the 16-entry hit list of the force kernels
+    fills and is evaluated over up to 5 and 8 passes."""
+    if which == "plain_10k":
+        spec, xyz, types = c1
+    elif which == "rich":
+        spec, xyz, types = rich_spec()
+    else:
+        spec, xyz, types = dense_spec() if which == "dense" else dense_spec(n=14000)
     o = orc.OracleSystem(spec)
     with Simulation(spec, xyz, types) as sim:
         g, e = sim.gradients()
