@@ -403,11 +403,23 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                     j1 = start[row + (run < 9 ? b0 : b1) + 1];
                 }
             };
+            // slots j .. j1-1 two at a time, the next two loaded while the
+            // current two are tested (the gather is bound by load latency;
+            // the same in the dense walk below measured slower on the 1M
+            // workload, whose register count it raises from 54 to 60)
             auto walk = [&](uint32_t j, uint32_t j1) {
-                for (; j + 1 < j1; j += 2) {
-                    const float4 q0 = ps[j], q1 = ps[j + 1];
+                if (j + 1 < j1) {
+                    float4 q0 = ps[j], q1 = ps[j + 1];
+                    for (; j + 3 < j1; j += 2) {
+                        const float4 n0 = ps[j + 2], n1 = ps[j + 3];
+                        test(j, q0);
+                        test(j + 1, q1);
+                        q0 = n0;
+                        q1 = n1;
+                    }
                     test(j, q0);
                     test(j + 1, q1);
+                    j += 2;
                 }
                 if (j < j1) test(j, ps[j]);
             };
