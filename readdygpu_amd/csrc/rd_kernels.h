@@ -324,6 +324,24 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
         if (P->S == 1 && Lx >= 3 && Ly >= 3 && Lz >= 3) {
             const bool in = interior_cell(R, cx, cy, cz);
             const int lane = threadIdx.x & 63;
+            // The walks address the particle array from a copy of its address
+            // in vector registers (an identity permutation across the lanes
+            // hides that it is uniform): held in scalar registers, it was
+            // evicted by the values the walks keep there and reloaded from
+            // the kernel arguments, with a wait, on every iteration.
+            typedef float rd_f4 __attribute__((ext_vector_type(4)));
+            typedef const __attribute__((address_space(1))) rd_f4* gptr;
+            gptr pva;
+            {
+                const uint64_t a = (uint64_t)(uintptr_t)ps;
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(lane << 2, (int)(uint32_t)a);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(lane << 2, (int)(uint32_t)(a >> 32));
+                pva = (gptr)(((uint64_t)hi << 32) | lo);
+            }
+            auto pv = [&](uint32_t j) {
+                const rd_f4 v = pva[j];
+                return make_float4(v.x, v.y, v.z, v.w);
+            };
             // Phase 1: every stencil candidate gets the cheap distance test
             // against the range of its type pair (one LDS table read); hits
             // are listed in LDS.
@@ -409,9 +427,9 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
             // workload, whose register count it raises from 54 to 60)
             auto walk = [&](uint32_t j, uint32_t j1) {
                 if (j + 1 < j1) {
-                    float4 q0 = ps[j], q1 = ps[j + 1];
+                    float4 q0 = pv(j), q1 = pv(j + 1);
                     for (; j + 3 < j1; j += 2) {
-                        const float4 n0 = ps[j + 2], n1 = ps[j + 3];
+                        const float4 n0 = pv(j + 2), n1 = pv(j + 3);
                         test(j, q0);
                         test(j + 1, q1);
                         q0 = n0;
@@ -421,7 +439,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                     test(j + 1, q1);
                     j += 2;
                 }
-                if (j < j1) test(j, ps[j]);
+                if (j < j1) test(j, pv(j));
             };
             int rnext = nruns;              // first run the listing-only walk did not reach
             bool stop = false;
@@ -467,18 +485,18 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                         uint32_t j = sb[k];
                         const uint32_t j1 = se[k];
                         for (; j + 1 < j1; j += 2) {
-                            const float4 q0 = ps[j], q1 = ps[j + 1];
+                            const float4 q0 = pv(j), q1 = pv(j + 1);
                             test_dense(j, q0);
                             test_dense(j + 1, q1);
                         }
-                        if (j < j1) test_dense(j, ps[j]);
+                        if (j < j1) test_dense(j, pv(j));
                     }
                 }
             }
             // Phase 2
             for (int k = 0; __any(k < nhit); ++k) {
                 if (k < nhit) {
-                    const float4 q = ps[il->j[k][lane]];
+                    const float4 q = pv(il->j[k][lane]);
                     float dx, dy, dz;
                     delta(q, dx, dy, dz);
                     pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), acc);
