@@ -131,6 +131,43 @@ __device__ __forceinline__ void for_stencil(const RdRep& R, const uint32_t* __re
     }
 }
 
+// for_stencil with each slot's record loaded one slot ahead of its use, so
+// that a load is in flight while the previous candidate is examined;
+// f(j, q) with q = ps[j]
+template <class F>
+__device__ __forceinline__ void for_stencil_q(const RdRep& R, const uint32_t* __restrict__ start,
+                                              const float4* __restrict__ ps, int cx, int cy, int cz, int S, F&& f)
+{
+    auto run = [&](uint32_t j, uint32_t j1) {
+        if (j >= j1) return;
+        float4 q = ps[j];
+        for (; j + 1 < j1; ++j) {
+            const float4 n = ps[j + 1];
+            f(j, q);
+            q = n;
+        }
+        f(j, q);
+    };
+    const int Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
+    int ylo, yhi, zlo, zhi;
+    bool fy = 2 * S + 1 >= Ly, fz = 2 * S + 1 >= Lz;
+    if (fy) { ylo = 0; yhi = Ly - 1; } else { ylo = -S; yhi = S; }
+    if (fz) { zlo = 0; zhi = Lz - 1; } else { zlo = -S; zhi = S; }
+    int a0, b0, a1, b1;
+    int nr = x_runs(cx, S, Lx, &a0, &b0, &a1, &b1);
+    for (int oz = zlo; oz <= zhi; ++oz) {
+        int zz = fz ? oz : cz + oz;
+        if (zz < 0) zz += Lz; else if (zz >= Lz) zz -= Lz;
+        for (int oy = ylo; oy <= yhi; ++oy) {
+            int yy = fy ? oy : cy + oy;
+            if (yy < 0) yy += Ly; else if (yy >= Ly) yy -= Ly;
+            int row = R.coff + (zz * Ly + yy) * Lx;
+            run(start[row + a0], start[row + b0 + 1]);
+            if (nr == 2) run(start[row + a1], start[row + b1 + 1]);
+        }
+    }
+}
+
 // Pair gradients are summed in 64-bit fixed point (2^-32 resolution): integer
 // addition is exact, so the sum does not depend on the order of particles
 // inside a cell, which the atomic counting sort leaves unspecified.  The step

@@ -533,9 +533,9 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf_lds(const float4* __restrict__
         int cx, cy, cz;
         cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
         const bool wrap = stencil_wraps(R, cx, cy, cz, S);
-        for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
+        for_stencil_q(R, start, ps, cx, cy, cz, S, [&](uint32_t j, float4 q) {
             if ((int)j == s) return;
-            const int h = rdf_bin(R, p, ps[j], ti, nt, rmax, rcut2, nbins, wrap);
+            const int h = rdf_bin(R, p, q, ti, nt, rmax, rcut2, nbins, wrap);
             if (h < 0) return;
             if (one) atomicAdd(&hist[h], 1u);
             else atomicAdd(&cnt[h], 1ull);
@@ -585,9 +585,9 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf_global(const float4* __restric
     int cx, cy, cz;
     cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
     const bool wrap = stencil_wraps(R, cx, cy, cz, S);
-    for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
+    for_stencil_q(R, start, ps, cx, cy, cz, S, [&](uint32_t j, float4 q) {
         if ((int)j == s) return;
-        const int h = rdf_bin(R, p, ps[j], ti, nt, rmax, rcut2, nbins, wrap);
+        const int h = rdf_bin(R, p, q, ti, nt, rmax, rcut2, nbins, wrap);
         if (h >= 0) atomicAdd(&cnt[h], 1ull);
     });
 }
@@ -627,8 +627,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ p
     const uint32_t o = EMIT ? cnt_off[gi] : 0u;
     const bool wrap = stencil_wraps(R, cx, cy, cz, S);
     uint32_t c = 0;
-    for_stencil(R, start, cx, cy, cz, S, [&](uint32_t j) {
-        float4 q = ps[j];
+    for_stencil_q(R, start, ps, cx, cy, cz, S, [&](uint32_t j, float4 q) {
         uint32_t jj = w_index(q.w);
         if (jj <= ii) return;
         float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
