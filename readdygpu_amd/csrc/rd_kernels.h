@@ -530,13 +530,19 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                     }
                 }
             }
-            // Phase 2
-            for (int k = 0; __any(k < nhit); ++k) {
-                if (k < nhit) {
-                    const float4 q = pv(il->j[k][lane]);
-                    float dx, dy, dz;
-                    delta(q, dx, dy, dz);
-                    pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), acc);
+            // Phase 2, the next listed partner loaded while one is evaluated
+            if (__any(nhit > 0)) {
+                float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (nhit > 0) q = pv(il->j[0][lane]);
+                for (int k = 0; __any(k < nhit); ++k) {
+                    float4 n = q;
+                    if (k + 1 < nhit) n = pv(il->j[k + 1][lane]);
+                    if (k < nhit) {
+                        float dx, dy, dz;
+                        delta(q, dx, dy, dz);
+                        pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), acc);
+                    }
+                    q = n;
                 }
             }
         } else {
