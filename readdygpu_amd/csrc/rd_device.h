@@ -241,6 +241,57 @@ __device__ __forceinline__ float pair_pre_fast(const RdPot2Dev& p, float r0, flo
     return pre;
 }
 
+// The sum of pair_pre_fast over the potentials of bit mask m (bit p:
+// potential p), in potential order, with the same FP32 operations on every
+// branch, so the same bits.  The potential index runs uniformly across the
+// wave (potentials no lane needs are skipped), so kind and subtype select
+// uniform branches, and a lane's conditions become selects instead of a
+// per-lane loop that branches on values read per lane.
+__device__ __forceinline__ float rd_uniform(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+template <bool ENERGY>
+__device__ __forceinline__ float pair_pre_wave(const RdPot2Dev* pot2, int npot, uint32_t m, float r0, float r,
+                                               float rinv, float* e)
+{
+    float pre = 0.0f;
+    const float u = r - r0;
+    npot = __builtin_amdgcn_readfirstlane(npot);
+    for (int pi = 0; pi < npot; ++pi) {
+        const bool has = (m >> pi) & 1u;
+        if (!__any(has)) continue;
+        // the record is the same for every lane: read whole, held in scalar registers
+        const RdPot2Dev p = pot2[pi];
+        const int kind = __builtin_amdgcn_readfirstlane(p.kind);
+        const float k = rd_uniform(p.k), cut = rd_uniform(p.cutoff);
+        const bool in = has & !(r > cut);
+        float v = 0.0f;
+        bool on = false;
+        if (kind == 1) {
+            const int sub = __builtin_amdgcn_readfirstlane(p.subtype);
+            on = in & ((sub == 1 & r > r0) | (sub == 2 & r < r0) | sub == 3);
+            v = k * u * rinv;
+        } else if (kind == 2) {
+            const float ri = rd_uniform(p.length) + r0;
+            on = in & (r < ri) & (r > r0);
+            v = k * u * (ri - r);
+        } else if (kind == 3) {
+            const float l = rd_uniform(p.length), wc = rd_uniform(p.wc);
+            const float hl = 0.5f * l;
+            const bool a = r < r0, b = r < r0 + hl;
+            on = in & (r < r0 + l);
+            const float c = a ? k : (b ? wc : -wc);
+            const float w = (a | b) ? u : u - l;
+            v = c * w * rinv;
+        }
+        pre += on ? v : 0.0f;
+        if (ENERGY && has) (void)pair_pre_fast<true>(p, r0, r, rinv, e);
+    }
+    return pre;
+}
+
 // |v| and 1/|v| from v_rsq_f32; (0, 0) for v = 0
 __device__ __forceinline__ void fast_len(float v2, float* len, float* inv)
 {

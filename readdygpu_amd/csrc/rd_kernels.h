@@ -210,6 +210,7 @@ struct RdTab {
     uint32_t mask[RD_MAXT * RD_MAXT];
     float a[RD_MAXT], b[RD_MAXT], prad[RD_MAXT];
     RdPot2Dev pot2[RD_MAXP2];
+    int npot;
 };
 
 // all threads of the block must call this (it ends with a barrier)
@@ -229,27 +230,25 @@ __device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
     const int nw = (int)(sizeof(RdPot2Dev) / 4) * RD_MAXP2;
     for (int k = threadIdx.x; k < nw; k += blockDim.x)
         reinterpret_cast<uint32_t*>(T.pot2)[k] = reinterpret_cast<const uint32_t*>(P->pot2)[k];
+    if (threadIdx.x == 0) T.npot = P->n_pot2;
     __syncthreads();
 }
 
 // Gradient contribution of q on p for the separation d = p - q (|d|^2 = d2),
 // added to the fixed-point sums.  Every force kernel evaluates a pair through
-// this one function, so all of them produce the same bits.
-template <bool ENERGY>
+// this one function, so all of them produce the same bits.  LISTED: the pair
+// already passed the same range test (0 < d2 < rc2) on the same d2.
+template <bool ENERGY, bool LISTED = false>
 __device__ __forceinline__ void pair_add(const RdTab& T, int rowbase, float dx, float dy, float dz, float d2, int tj,
                                          RdAcc& acc)
 {
-    if (d2 < T.rc2[rowbase + tj] && d2 > 0.0f) {
-        float rinv = __builtin_amdgcn_rsqf(d2);
-        float r = d2 * rinv;
-        float r0 = T.r0[rowbase + tj];
-        uint32_t m = T.mask[rowbase + tj];
-        float pre = 0.0f, ep = 0.0f;
-        while (m) {
-            int pi = __builtin_ctz(m);
-            m &= m - 1;
-            pre += pair_pre_fast<ENERGY>(T.pot2[pi], r0, r, rinv, &ep);
-        }
+    if (LISTED || (d2 < T.rc2[rowbase + tj] && d2 > 0.0f)) {
+        const float rinv = __builtin_amdgcn_rsqf(d2);
+        const float r = d2 * rinv;
+        const float r0 = T.r0[rowbase + tj];
+        const uint32_t m = T.mask[rowbase + tj];
+        float ep = 0.0f;
+        const float pre = pair_pre_wave<ENERGY>(T.pot2, T.npot, m, r0, r, rinv, &ep);
         acc_add(acc, pre, dx, dy, dz);
         if (ENERGY) acc.e += (double)(0.5f * ep);
     }
@@ -436,7 +435,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                         il->j[nhit][lane] = j;
                         ++nhit;
                     } else {
-                        pair_add<ENERGY>(T, rowbase, dx, dy, dz, d2, w_type(q.w), acc);
+                        pair_add<ENERGY, true>(T, rowbase, dx, dy, dz, d2, w_type(q.w), acc);
                     }
                 }
             };
@@ -540,7 +539,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                     if (k < nhit) {
                         float dx, dy, dz;
                         delta(q, dx, dy, dz);
-                        pair_add<ENERGY>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), acc);
+                        pair_add<ENERGY, true>(T, rowbase, dx, dy, dz, dist2(dx, dy, dz), w_type(q.w), acc);
                     }
                     q = n;
                 }
