@@ -175,16 +175,6 @@ __device__ __forceinline__ void for_stencil_q(const RdRep& R, const uint32_t* __
 #define RD_FIX_SCALE 4294967296.0f
 #define RD_FIX_INV 2.3283064365386963e-10
 #define RD_FIX_MAX 5.0e8f        // |term| limit; a larger term is clamped and reported
-__device__ __forceinline__ long long to_fix(float v)
-{
-    // floor(v * 2^32) from the integer part and the exact fraction: two
-    // 32-bit conversions instead of a float -> int64 conversion sequence
-    v = fminf(fmaxf(v, -RD_FIX_MAX), RD_FIX_MAX);
-    const float fl = floorf(v);
-    const int hi = (int)fl;
-    const uint32_t lo = (uint32_t)((v - fl) * RD_FIX_SCALE);
-    return (long long)(((unsigned long long)(uint32_t)hi << 32) | lo);
-}
 
 // Per-particle accumulator: fixed-point gradient sums, FP64 energy, and
 // whether any term was clamped to the fixed-point range (reported by the
@@ -195,11 +185,27 @@ struct RdAcc {
     bool sat;
 };
 
+// s += floor(v * 2^32), clamped to the range: the integer part and the exact
+// fraction from two 32-bit conversions (instead of a float -> int64
+// conversion sequence), added as the low word and, with its carry, the high
+// word
+__device__ __forceinline__ void fix_add(long long& s, float v)
+{
+    v = fminf(fmaxf(v, -RD_FIX_MAX), RD_FIX_MAX);
+    const float fl = floorf(v);
+    const uint32_t hi = (uint32_t)(int)fl;
+    const uint32_t lo = (uint32_t)((v - fl) * RD_FIX_SCALE);
+    unsigned int c;
+    const uint32_t sl = __builtin_addc((uint32_t)s, lo, 0u, &c);
+    const uint32_t sh = __builtin_addc((uint32_t)((unsigned long long)s >> 32), hi, c, &c);
+    s = (long long)(((unsigned long long)sh << 32) | sl);
+}
+
 __device__ __forceinline__ void acc_add(RdAcc& a, float pre, float dx, float dy, float dz)
 {
     const float fx = pre * dx, fy = pre * dy, fz = pre * dz;
     a.sat = a.sat || !(fabsf(fx) < RD_FIX_MAX && fabsf(fy) < RD_FIX_MAX && fabsf(fz) < RD_FIX_MAX);
-    a.x += to_fix(fx); a.y += to_fix(fy); a.z += to_fix(fz);
+    fix_add(a.x, fx); fix_add(a.y, fy); fix_add(a.z, fz);
 }
 
 // Per-block LDS copy of the type-pair tables and EM coefficients: the inner
@@ -461,18 +467,22 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
             // current two are tested (the gather is bound by load latency;
             // the same in the dense walk below measured slower on the 1M
             // workload, whose register count it raises from 54 to 60)
+            // (the loads step a pointer, so that each pair of candidates costs
+            // one address update instead of two index-to-address conversions)
             auto walk = [&](uint32_t j, uint32_t j1) {
                 if (j + 1 < j1) {
-                    float4 q0 = pv(j), q1 = pv(j + 1);
+                    gptr a = pva + j;
+                    rd_f4 q0 = a[0], q1 = a[1];
                     for (; j + 3 < j1; j += 2) {
-                        const float4 n0 = pv(j + 2), n1 = pv(j + 3);
-                        test(j, q0);
-                        test(j + 1, q1);
+                        a += 2;
+                        const rd_f4 n0 = a[0], n1 = a[1];
+                        test(j, make_float4(q0.x, q0.y, q0.z, q0.w));
+                        test(j + 1, make_float4(q1.x, q1.y, q1.z, q1.w));
                         q0 = n0;
                         q1 = n1;
                     }
-                    test(j, q0);
-                    test(j + 1, q1);
+                    test(j, make_float4(q0.x, q0.y, q0.z, q0.w));
+                    test(j + 1, make_float4(q1.x, q1.y, q1.z, q1.w));
                     j += 2;
                 }
                 if (j < j1) test(j, pv(j));
