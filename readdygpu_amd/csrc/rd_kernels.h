@@ -168,6 +168,43 @@ __device__ __forceinline__ void for_stencil_q(const RdRep& R, const uint32_t* __
     }
 }
 
+// for_stencil_q over the half of the stencil that holds each unordered pair
+// once: the rows (dy, dz) with dz > 0, or dz == 0 and dy > 0, in full, and of
+// the home row the slots after s.  The mirror image of every other row is
+// left out, so a pair is met from exactly one of its two particles.  Needs
+// 2S+1 < L in every dimension (no stencil row is its own mirror image).
+template <class F>
+__device__ __forceinline__ void for_stencil_half_q(const RdRep& R, const uint32_t* __restrict__ start,
+                                                   const float4* __restrict__ ps, uint32_t s, int cx, int cy, int cz,
+                                                   int S, F&& f)
+{
+    auto run = [&](uint32_t j, uint32_t j1) {
+        if (j >= j1) return;
+        float4 q = ps[j];
+        for (; j + 1 < j1; ++j) {
+            const float4 n = ps[j + 1];
+            f(j, q);
+            q = n;
+        }
+        f(j, q);
+    };
+    const int Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
+    int a0, b0, a1, b1;
+    const int nr = x_runs(cx, S, Lx, &a0, &b0, &a1, &b1);
+    for (int oz = 0; oz <= S; ++oz) {
+        int zz = cz + oz;
+        if (zz >= Lz) zz -= Lz;
+        for (int oy = oz == 0 ? 0 : -S; oy <= S; ++oy) {
+            int yy = cy + oy;
+            if (yy < 0) yy += Ly; else if (yy >= Ly) yy -= Ly;
+            const int row = R.coff + (zz * Ly + yy) * Lx;
+            const uint32_t lo = (oz == 0 && oy == 0) ? s + 1u : 0u;
+            run(max(start[row + a0], lo), start[row + b0 + 1]);
+            if (nr == 2) run(max(start[row + a1], lo), start[row + b1 + 1]);
+        }
+    }
+}
+
 // Pair gradients are summed in 64-bit fixed point (2^-32 resolution): integer
 // addition is exact, so the sum does not depend on the order of particles
 // inside a cell, which the atomic counting sort leaves unspecified.  The step

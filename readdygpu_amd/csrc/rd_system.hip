@@ -468,8 +468,9 @@ __global__ __launch_bounds__(RD_BLOCK) void k_noise_fill(const RdDev* __restrict
 // (which also holds the partial slab at the top of the box), and it spans
 // less than half the box, so every raw difference is already its minimum
 // image (min_image would return it unchanged).
-__device__ __forceinline__ int rdf_bin(const RdRep& R, float4 p, float4 q, int ti, int nt, float rmax, float rcut2,
-                                       int nbins, bool wrap)
+// bin of the pair (p, q), or -1 beyond rmax
+__device__ __forceinline__ int rdf_bin(const RdRep& R, float4 p, float4 q, float rmax, float rcut2, int nbins,
+                                       bool wrap)
 {
     float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     if (wrap) {
@@ -485,7 +486,7 @@ __device__ __forceinline__ int rdf_bin(const RdRep& R, float4 p, float4 q, int t
     if (!(r < rmax)) return -1;
     int bin = (int)floorf((r * (float)nbins) / rmax);
     if (bin > nbins - 1) bin = nbins - 1;
-    return (ti * nt + w_type(q.w)) * nbins + bin;
+    return bin;
 }
 
 __device__ __forceinline__ bool stencil_wraps(const RdRep& R, int cx, int cy, int cz, int S)
@@ -494,6 +495,43 @@ __device__ __forceinline__ bool stencil_wraps(const RdRep& R, int cx, int cy, in
     for (int d = 0; d < 3; ++d)
         if (c[d] < S + 1 || c[d] > L[d] - 1 - S || (float)(S + 2) * R.cell >= R.half[d]) return true;
     return false;
+}
+
+// Visits every pair of the particle in slot s with the RDF histogram index
+// of each direction in which the full-stencil count (every particle counting
+// its partners) meets it: add(h, c) with c = 1, or 2 where both directions
+// share the index (equal types).  Where the lattice allows, the half stencil
+// meets each pair once and adds both directions, half the candidates.
+template <class F>
+__device__ __forceinline__ void rdf_pairs(const RdRep& R, const uint32_t* __restrict__ start,
+                                          const float4* __restrict__ ps, int s, float4 p, int nt, int S, float rmax,
+                                          float rcut2, int nbins, F&& add)
+{
+    const int ti = w_type(p.w);
+    int cx, cy, cz;
+    cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
+    const bool wrap = stencil_wraps(R, cx, cy, cz, S);
+    const bool half = 2 * S + 1 < R.L[0] && 2 * S + 1 < R.L[1] && 2 * S + 1 < R.L[2];
+    if (half) {
+        for_stencil_half_q(R, start, ps, (uint32_t)s, cx, cy, cz, S, [&](uint32_t j, float4 q) {
+            const int b = rdf_bin(R, p, q, rmax, rcut2, nbins, wrap);
+            if (b < 0) return;
+            const int tj = w_type(q.w);
+            const int h1 = (ti * nt + tj) * nbins + b, h2 = (tj * nt + ti) * nbins + b;
+            if (h1 == h2) {
+                add(h1, 2u);
+            } else {
+                add(h1, 1u);
+                add(h2, 1u);
+            }
+        });
+    } else {
+        for_stencil_q(R, start, ps, cx, cy, cz, S, [&](uint32_t j, float4 q) {
+            if ((int)j == s) return;
+            const int b = rdf_bin(R, p, q, rmax, rcut2, nbins, wrap);
+            if (b >= 0) add((ti * nt + w_type(q.w)) * nbins + b, 1u);
+        });
+    }
 }
 
 // One LDS histogram per workgroup, shared by its waves when their work items
@@ -529,16 +567,9 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf_lds(const float4* __restrict__
         const RdRep R = reps[it.rep];
         unsigned long long* cnt = counts + (size_t)it.rep * nh;
         const float4 p = ps[s];
-        const int ti = w_type(p.w);
-        int cx, cy, cz;
-        cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
-        const bool wrap = stencil_wraps(R, cx, cy, cz, S);
-        for_stencil_q(R, start, ps, cx, cy, cz, S, [&](uint32_t j, float4 q) {
-            if ((int)j == s) return;
-            const int h = rdf_bin(R, p, q, ti, nt, rmax, rcut2, nbins, wrap);
-            if (h < 0) return;
-            if (one) atomicAdd(&hist[h], 1u);
-            else atomicAdd(&cnt[h], 1ull);
+        rdf_pairs(R, start, ps, s, p, nt, S, rmax, rcut2, nbins, [&](int h, uint32_t c) {
+            if (one) atomicAdd(&hist[h], c);
+            else atomicAdd(&cnt[h], (unsigned long long)c);
         });
     }
     __syncthreads();
@@ -581,15 +612,8 @@ __global__ __launch_bounds__(RD_BLOCK) void k_rdf_global(const float4* __restric
     const RdRep R = reps[it.rep];
     unsigned long long* cnt = counts + (size_t)it.rep * nh;
     const float4 p = ps[s];
-    const int ti = w_type(p.w);
-    int cx, cy, cz;
-    cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
-    const bool wrap = stencil_wraps(R, cx, cy, cz, S);
-    for_stencil_q(R, start, ps, cx, cy, cz, S, [&](uint32_t j, float4 q) {
-        if ((int)j == s) return;
-        const int h = rdf_bin(R, p, q, ti, nt, rmax, rcut2, nbins, wrap);
-        if (h >= 0) atomicAdd(&cnt[h], 1ull);
-    });
+    rdf_pairs(R, start, ps, s, p, nt, S, rmax, rcut2, nbins,
+              [&](int h, uint32_t c) { atomicAdd(&cnt[h], (unsigned long long)c); });
 }
 
 // pairs (i<j by index) with d2 < thr2[ti][tj]; global particle indices
