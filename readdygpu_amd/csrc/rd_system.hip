@@ -646,6 +646,14 @@ __global__ __launch_bounds__(RD_BLOCK) void k_pairs(const float4* __restrict__ p
     int ti = w_type(p.w);
     uint32_t ii = w_index(p.w);
     const uint32_t gi = (uint32_t)R.off + ii;
+    // a type with no partner type (every threshold 0, as for the species
+    // that take part in no reaction) has no pairs: no walk
+    float tmax = 0.0f;
+    for (int k = 0; k < nt; ++k) tmax = fmaxf(tmax, thr2[ti * nt + k]);
+    if (!(tmax > 0.0f)) {
+        if (!EMIT) cnt_off[gi] = 0u;
+        return;
+    }
     int cx, cy, cz;
     cell_coords(R, p.x, p.y, p.z, cx, cy, cz);
     const uint32_t o = EMIT ? cnt_off[gi] : 0u;
