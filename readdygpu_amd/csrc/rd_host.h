@@ -121,6 +121,9 @@ struct rd_system {
     // pair emission
     uint2* d_pairs = nullptr;
     size_t pairs_cap = 0;
+    uint32_t* d_poff = nullptr;   // per-particle pair offsets and their scan tile sums
+    uint32_t* d_pbs = nullptr;
+    size_t poff_cap = 0;
 
     // captured steps: one graph per step count
     struct Graph {

@@ -759,6 +759,7 @@ static void free_dev(rd_system* h)
     free_frames(h);
     void* ptrs[] = { h->d_ps, h->d_pn, h->d_kr, h->d_count, h->d_start, h->d_bsum,
                      h->d_P, h->d_pot1, h->d_st, h->d_noise, h->d_rdf, h->d_rdf_part, h->d_pairs,
+                     h->d_poff, h->d_pbs,
                      h->d_xyz_io, h->d_type_io, h->d_etot, h->d_reps, h->d_waves, h->d_rep_of };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -767,6 +768,7 @@ static void free_dev(rd_system* h)
     h->d_count = h->d_start = h->d_bsum = nullptr;
     h->d_P = nullptr; h->d_pot1 = nullptr; h->d_st = nullptr; h->d_noise = nullptr;
     h->d_rdf = nullptr; h->d_rdf_part = nullptr; h->d_pairs = nullptr;
+    h->d_poff = nullptr; h->d_pbs = nullptr; h->poff_cap = 0;
     h->d_xyz_io = nullptr; h->d_type_io = nullptr; h->d_etot = nullptr;
     h->d_reps = nullptr; h->d_waves = nullptr; h->d_rep_of = nullptr;
     h->cap = 0; h->ncell_alloc = 0; h->pairs_cap = 0; h->noise_cap = 0; h->nrep_alloc = 0; h->nwave_alloc = 0;
@@ -1779,12 +1781,23 @@ static int pairs_common(rd_system* h, bool reactions, int64_t* out, long cap, lo
     for (int a = 0; a < h->nt * h->nt; ++a) m2 = std::max(m2, table[a]);
     if (h->n == 0 || m2 <= 0.0f) { if (count) *count = 0; return RD_OK; }
     int S = std::max(1, (int)std::ceil(std::sqrt((double)m2) * 1.0001 / h->cell));
-    // per-particle counts -> offsets (the counting-sort scan over n entries)
+    // per-particle counts -> offsets (the counting-sort scan over n entries),
+    // in buffers the handle keeps between calls
     const int n = h->n;
     const int nsc = nblk(n, RD_SCAN_TILE);
-    uint32_t *d_off = nullptr, *d_bs = nullptr;
-    hipError_t e = hipMalloc(&d_off, sizeof(uint32_t) * ((size_t)n + 16));
-    if (e == hipSuccess) e = hipMalloc(&d_bs, sizeof(uint32_t) * ((size_t)nsc + 16));
+    hipError_t e = hipSuccess;
+    if ((size_t)n + 16 > h->poff_cap) {
+        for (void* p : { (void*)h->d_poff, (void*)h->d_pbs })
+            if (p) (void)hipFree(p);
+        h->d_poff = nullptr;
+        h->d_pbs = nullptr;
+        h->poff_cap = 0;
+        e = hipMalloc(&h->d_poff, sizeof(uint32_t) * ((size_t)n + 16));
+        if (e == hipSuccess) e = hipMalloc(&h->d_pbs, sizeof(uint32_t) * ((size_t)nblk((long)n + 16, RD_SCAN_TILE) + 16));
+        if (e == hipSuccess) h->poff_cap = (size_t)n + 16;
+    }
+    uint32_t* d_off = h->d_poff;
+    uint32_t* d_bs = h->d_pbs;
     uint32_t total = 0;
     if (e == hipSuccess) launch_pair_counts(h, dtable, S, d_off, d_bs);
     if (e == hipSuccess) e = hipGetLastError();
@@ -1807,8 +1820,6 @@ static int pairs_common(rd_system* h, bool reactions, int64_t* out, long cap, lo
             e = hipMemcpyAsync(raw.data(), h->d_pairs, sizeof(uint2) * (size_t)nout, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     }
-    for (void* p : { (void*)d_off, (void*)d_bs })
-        if (p) (void)hipFree(p);
     if (e != hipSuccess) return set_err(h, RD_EHIP, std::string("pair emission: ") + hipGetErrorString(e));
     for (long k = 0; k < nout; ++k) { out[2 * k] = (int64_t)raw[k].x; out[2 * k + 1] = (int64_t)raw[k].y; }
     if (count) *count = (long)total;
