@@ -442,9 +442,8 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
             // registers free (inlined there, it made the compiler reload the
             // particle array's address from the kernel arguments on every
             // iteration).
-            int nhit = 0;
-            bool over = false;              // list full: (res_run, res_j) is the first unlisted hit
-            int cur_run = 0, res_run = 0;
+            int nhit = 0;                   // hits met; more than RD_HITS: the list is full
+            int cur_run = 0, res_run = 0;   // (res_run, res_j): the first unlisted hit
             uint32_t res_j = 0u;
             auto delta = [&](float4 q, float& dx, float& dy, float& dz) {
                 dx = p.x - q.x; dy = p.y - q.y; dz = p.z - q.z;
@@ -459,14 +458,11 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                 delta(q, dx, dy, dz);
                 const float d2 = dist2(dx, dy, dz);
                 if (d2 < T.rc2[rowbase + w_type(q.w)] && d2 > 0.0f) {
-                    if (nhit < RD_HITS) {
-                        il->j[nhit][lane] = j;
-                        ++nhit;
-                    } else if (!over) {
-                        over = true;
-                        res_run = cur_run;
-                        res_j = j;
-                    }
+                    if (nhit < RD_HITS) il->j[nhit][lane] = j;
+                    const bool first = nhit == RD_HITS;
+                    res_j = first ? j : res_j;
+                    res_run = first ? cur_run : res_run;
+                    ++nhit;
                 }
             };
             auto test_dense = [&](uint32_t j, float4 q) {     // listing, then evaluating
@@ -495,9 +491,15 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
             auto bounds = [&](int run, uint32_t& j0, uint32_t& j1) {
                 j0 = 0u; j1 = 0u;
                 if (run < 9 || nr == 2) {
+                    // both bounds from one 16-byte load: a run is at most
+                    // three cells, so its end is one of the next three
+                    // starts (the start array has 16 entries of padding)
                     const int row = row_of(run < 9 ? run : run - 9);
-                    j0 = start[row + (run < 9 ? a0 : a1)];
-                    j1 = start[row + (run < 9 ? b0 : b1) + 1];
+                    const int a = run < 9 ? a0 : a1, len = (run < 9 ? b0 : b1) + 1 - a;
+                    uint4 v;
+                    __builtin_memcpy(&v, start + row + a, sizeof(v));
+                    j0 = v.x;
+                    j1 = len == 3 ? v.w : (len == 2 ? v.z : v.y);
                 }
             };
             // slots j .. j1-1 two at a time, the next two loaded while the
@@ -535,7 +537,7 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
                         if (!stop) {
-                            if (__any(over)) {
+                            if (__any(nhit > RD_HITS)) {
                                 stop = true;
                                 rnext = g + k;
                             } else {
@@ -546,6 +548,8 @@ __device__ __forceinline__ float3 gather_grad(const RdDev* __restrict__ P, const
                     }
                 }
             }
+            const bool over = nhit > RD_HITS;
+            nhit = min(nhit, RD_HITS);
             if (__any(over)) {
                 // go on from the run in which lists filled (rnext - 1, the
                 // last one walked): lanes whose list filled from their first
