@@ -66,12 +66,14 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
         if (user_lc.block == 0 || user_lc.block == b) cfgs.push_back({ k0, b, ppt0 });
     // the default shape (256 threads, one particle per lane, or what the
     // caller fixed) is kept unless another shape is more than 2% faster, and
-    // the smallest cell unless a larger one is more than 2% faster: the
+    // the smallest cell unless a larger one is more than 5% faster: the
     // trials are short and taken on the current state, and a few percent is
     // within what the state's evolution over a run moves.  The bias goes to
     // the smallest cell because a larger cell's candidate count grows with
     // its volume as the local density rises (on the 1M workload 7.2 nm ties
-    // 6.06 nm over the first steps and is 10% slower over 500).
+    // 6.06 nm over the first steps and is 10% slower over 500; on the
+    // learned slab a 3-step trial after 10 steps has picked the larger cell,
+    // whose force kernel then ran 40% slower over the next 100 steps).
     const int def_block = user_lc.block ? user_lc.block : 256;
     const Cfg def_cfg{ k0, def_block, ppt0 };
     const size_t nc = cells.size();
@@ -108,7 +110,7 @@ extern "C" int rd_autotune(rd_system* h, int trial_steps, double* best_ms)
     size_t cb = 0;
     for (size_t ci = 1; ci < nc; ++ci)
         if (t_best[ci] < t_best[cb]) cb = ci;
-    if (cb != 0 && !(t_best[cb] < 0.98 * t_best[0])) cb = 0;
+    if (cb != 0 && !(t_best[cb] < 0.95 * t_best[0])) cb = 0;
     Cfg best_cfg = k_best[cb];
     double best = t_best[cb];
     if (t_def[cb] < 1e30 && !(best < 0.98 * t_def[cb])) {
