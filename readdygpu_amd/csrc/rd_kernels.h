@@ -256,24 +256,47 @@ struct RdTab {
     int npot;
 };
 
-// all threads of the block must call this (it ends with a barrier)
+// all threads of the block must call this (it ends with a barrier).  BLOCK
+// is the workgroup size: the loop strides are constants (blockDim.x would be
+// read from the dispatch packet, a dependent load before each loop), and
+// every thread issues all its loads before its LDS stores, so that the table
+// costs one global latency at the start of a workgroup instead of several.
+template <int BLOCK>
 __device__ __forceinline__ void load_tab(RdTab& T, const RdDev* __restrict__ P)
 {
+    constexpr int NP = (RD_MAXT * RD_MAXT + BLOCK - 1) / BLOCK;
+    constexpr int NW = (int)(sizeof(RdPot2Dev) / 4) * RD_MAXP2;
+    constexpr int NQ = (NW + BLOCK - 1) / BLOCK;
+    const int t = threadIdx.x;
     const int n2 = P->nt * P->nt;
-    for (int k = threadIdx.x; k < n2; k += blockDim.x) {
-        T.rc2[k] = P->rc2[k];
-        T.r0[k] = P->r0[k];
-        T.mask[k] = P->p2mask[k];
+    float rc2[NP], r0[NP];
+    uint32_t mk[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const int k = t + i * BLOCK;
+        if (k < n2) { rc2[i] = P->rc2[k]; r0[i] = P->r0[k]; mk[i] = P->p2mask[k]; }
     }
-    for (int k = threadIdx.x; k < RD_MAXT; k += blockDim.x) {
-        T.a[k] = P->a[k];
-        T.b[k] = P->b[k];
-        T.prad[k] = P->prad[k];
+    float a = 0.0f, b = 0.0f, prad = 0.0f;
+    if (t < RD_MAXT) { a = P->a[t]; b = P->b[t]; prad = P->prad[t]; }
+    uint32_t pw[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int k = t + i * BLOCK;
+        if (k < NW) pw[i] = reinterpret_cast<const uint32_t*>(P->pot2)[k];
     }
-    const int nw = (int)(sizeof(RdPot2Dev) / 4) * RD_MAXP2;
-    for (int k = threadIdx.x; k < nw; k += blockDim.x)
-        reinterpret_cast<uint32_t*>(T.pot2)[k] = reinterpret_cast<const uint32_t*>(P->pot2)[k];
-    if (threadIdx.x == 0) T.npot = P->n_pot2;
+    const int np = P->n_pot2;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const int k = t + i * BLOCK;
+        if (k < n2) { T.rc2[k] = rc2[i]; T.r0[k] = r0[i]; T.mask[k] = mk[i]; }
+    }
+    if (t < RD_MAXT) { T.a[t] = a; T.b[t] = b; T.prad[t] = prad; }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int k = t + i * BLOCK;
+        if (k < NW) reinterpret_cast<uint32_t*>(T.pot2)[k] = pw[i];
+    }
+    if (t == 0) T.npot = np;
     __syncthreads();
 }
 

@@ -59,18 +59,23 @@ __device__ __forceinline__ void force_update(RD_FORCE_ARGS)
 {
     __shared__ RdTab T;
     __shared__ RdHitList il[BLOCK / 64];
-    load_tab(T, P);
     const int w = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64) + (threadIdx.x >> 6)) * PPT;
     if constexpr (PPT == 1) {
         const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w));
         const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
-        if (s >= (uint32_t)it.s1) return;
+        const bool act = s < (uint32_t)it.s1;
+        // the particle's record is loaded before the table, so that the two
+        // loads' latencies overlap instead of following each other
+        float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (act) p = ps[s];
+        load_tab<BLOCK>(T, P);
+        if (!act) return;
         const RdRep R = reps[it.rep];
-        float4 p = ps[s];
         bool sat;
         float3 g = gather_grad<false>(P, R, T, pot1, ps, start, s, p, nullptr, &sat, &il[threadIdx.x >> 6]);
         em_update_store(P, R, T, st, noise, ntot, s, p, g, sat, pn, kr, count, tsum);
     } else {
+        load_tab<BLOCK>(T, P);
         for (int k = 0; k < PPT; ++k) {
             const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(w + k));
             const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
@@ -117,7 +122,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void
     __shared__ RdHitList il[BLOCK / 64];
     __shared__ unsigned int ticket;
     if (threadIdx.x == 0) ticket = 0u;
-    load_tab(T, P);                         // ends with a workgroup barrier
+    load_tab<BLOCK>(T, P);                  // ends with a workgroup barrier
     const int lane = threadIdx.x & 63;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int per = (nwaves + (int)gridDim.x - 1) / (int)gridDim.x;
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_gradients(const float4* __restrict
     __shared__ double red[RD_BLOCK / 64];
     __shared__ RdTab T;
     __shared__ RdHitList il[RD_BLOCK / 64];
-    load_tab(T, P);
+    load_tab<RD_BLOCK>(T, P);
     const RdWave it = wave_item(waves, nwaves, __builtin_amdgcn_readfirstlane(wave_id()));
     const uint32_t s = (uint32_t)(it.s0 + (threadIdx.x & 63));
     double e = 0.0;
