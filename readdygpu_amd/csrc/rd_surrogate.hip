@@ -635,22 +635,39 @@ struct RdMlpFwdLds {
     float w1[192], b1[64], b2[64], b3[64], b4[4];
 };
 
+// BLOCK threads stage the weights with compile-time strides (blockDim.x
+// would be a load from the dispatch packet ahead of each loop), loads issued
+// in two batches ahead of their LDS stores (two global latencies instead of
+// one per loop iteration; one batch of everything spills registers)
+template <int BLOCK>
 __device__ __forceinline__ void mlp_stage(RdMlpFwdLds& L, const RdMlpPack* __restrict__ pk_g)
 {
     const uint4* src = reinterpret_cast<const uint4*>(pk_g);      // w2, w3 lead the pack
     uint4* dst = reinterpret_cast<uint4*>(&L);
-    const int n23 = (int)(sizeof(rd_bf16x8) * (2 * 4 * 2 * 64) / 16);
-    for (int i = threadIdx.x; i < n23; i += blockDim.x) dst[i] = src[i];
+    constexpr int N23 = (int)(sizeof(rd_bf16x8) * (2 * 4 * 2 * 64) / 16);
+    static_assert(N23 % (2 * BLOCK) == 0 && BLOCK >= 192, "staging layout");
+    const int t = threadIdx.x;
     const uint4* s4 = reinterpret_cast<const uint4*>(pk_g->w4s);
     uint4* d4 = reinterpret_cast<uint4*>(L.w4);
-    for (int i = threadIdx.x; i < 2 * 64; i += blockDim.x) d4[i] = s4[i];
-    for (int i = threadIdx.x; i < 192; i += blockDim.x) L.w1[i] = pk_g->w1s[i];
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
-        L.b1[i] = pk_g->b1s[i];
-        L.b2[i] = pk_g->b2s[i];
-        L.b3[i] = pk_g->b3s[i];
+    // batch 1: the small arrays and half of w2, w3
+    uint4 v4 = make_uint4(0u, 0u, 0u, 0u);
+    float v1 = 0.0f, vb1 = 0.0f, vb2 = 0.0f, vb3 = 0.0f, vb4 = 0.0f;
+    if (t < 2 * 64) v4 = s4[t];
+    if (t < 192) v1 = pk_g->w1s[t];
+    if (t < 64) { vb1 = pk_g->b1s[t]; vb2 = pk_g->b2s[t]; vb3 = pk_g->b3s[t]; }
+    if (t < 4) vb4 = pk_g->b4[t];
+    constexpr int KH = N23 / (2 * BLOCK);
+    for (int h = 0; h < 2; ++h) {
+        uint4 v[KH];
+#pragma unroll
+        for (int k = 0; k < KH; ++k) v[k] = src[t + (h * KH + k) * BLOCK];
+#pragma unroll
+        for (int k = 0; k < KH; ++k) dst[t + (h * KH + k) * BLOCK] = v[k];
     }
-    if (threadIdx.x < 4) L.b4[threadIdx.x] = pk_g->b4[threadIdx.x];
+    if (t < 2 * 64) d4[t] = v4;
+    if (t < 192) L.w1[t] = v1;
+    if (t < 64) { L.b1[t] = vb1; L.b2[t] = vb2; L.b3[t] = vb3; }
+    if (t < 4) L.b4[t] = vb4;
 }
 
 // -log2(e) SiLU(z) from zs = -log2(e) z
@@ -719,7 +736,7 @@ __global__ __launch_bounds__(256) void k_mlp_slots(const RdMlpPack* __restrict__
 {
     __shared__ RdMlpFwdLds L;
     __shared__ uint32_t ring[4][RD_MLP_RING];
-    mlp_stage(L, pk);
+    mlp_stage<256>(L, pk);
     __syncthreads();
     const int lane = threadIdx.x & 63, j = lane & 15;
     uint32_t* q = ring[threadIdx.x >> 6];
@@ -759,7 +776,7 @@ __global__ __launch_bounds__(256) void k_mlp_points(const RdMlpPack* __restrict_
                                                     long m, float4* __restrict__ out)
 {
     __shared__ RdMlpFwdLds L;
-    mlp_stage(L, pk);
+    mlp_stage<256>(L, pk);
     __syncthreads();
     const int lane = threadIdx.x & 63, j = lane & 15;
     const long ntiles = (m + 15) / 16;
