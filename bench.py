@@ -268,7 +268,7 @@ def bench_learned(steps=100, warmup=10, train_steps=3000, samples=4_000_000, dir
     # start, seed and step counter running the surrogate: identical noise; the
     # surrogate handle runs the lattice and launch shape tuned for the exact
     # one, so that only the external pass differs
-    wall_e, force_e, sort_e, ext_e = bench_handle(sim, steps, warmup, tune_steps=3, direct=direct)
+    wall_e, force_e, sort_e, ext_e = bench_handle(sim, steps, warmup, tune_steps=10, direct=direct)
     cfg = tuned_config(sim)
     sim.call_rdf_calculation()
     g_exact = sim.normalize_rdf()
