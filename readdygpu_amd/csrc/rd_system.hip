@@ -342,21 +342,42 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_offsets(uint32_t* v, int n, c
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) v[n] = pre + bsum[blockIdx.x];
 }
 
-// counting-sort scatter; advances the step counter when `advance` is set
+// counting-sort scatter; advances the step counter when `advance` is set.
+// Each thread moves RD_SCATTER_PPL records RD_BLOCK slots apart (every load
+// and store instruction covers contiguous slots), with all of its key and
+// record loads, then all of its start lookups, in flight at once.
+#define RD_SCATTER_PPL 4
 __global__ __launch_bounds__(RD_BLOCK) void k_scatter(const float4* __restrict__ pn, const uint2* __restrict__ kr,
                                                       const uint32_t* __restrict__ start, float4* __restrict__ ps,
                                                       int n, RdStepState* __restrict__ st, int advance,
                                                       const uint32_t* __restrict__ gid_n, uint32_t* __restrict__ slot_of,
                                                       uint32_t* __restrict__ tsum, int ntile)
 {
-    int s = blockIdx.x * RD_BLOCK + threadIdx.x;
-    for (int t = s; t < ntile; t += gridDim.x * RD_BLOCK) tsum[t] = 0u;
-    if (advance && s == 0) st->step += 1ull;
-    if (s >= n) return;
-    const uint2 q = kr[s];
-    const uint32_t dst = start[q.x] + q.y;
-    ps[dst] = pn[s];
-    if (slot_of) slot_of[gid_n[s]] = dst;
+    const int tid = blockIdx.x * RD_BLOCK + threadIdx.x;
+    for (int t = tid; t < ntile; t += gridDim.x * RD_BLOCK) tsum[t] = 0u;
+    if (advance && tid == 0) st->step += 1ull;
+    if (n <= 0) return;
+    // slots past the end are read and moved as slot n - 1 (the same record
+    // to the same place, a duplicate of that slot's own store), so that no
+    // load or store waits on a branch
+    const int s0 = blockIdx.x * (RD_BLOCK * RD_SCATTER_PPL) + threadIdx.x;
+    uint2 q[RD_SCATTER_PPL];
+    float4 v[RD_SCATTER_PPL];
+#pragma unroll
+    for (int u = 0; u < RD_SCATTER_PPL; ++u) {
+        const int s = min(s0 + u * RD_BLOCK, n - 1);
+        q[u] = kr[s];
+        v[u] = pn[s];
+    }
+    uint32_t d[RD_SCATTER_PPL];
+#pragma unroll
+    for (int u = 0; u < RD_SCATTER_PPL; ++u) d[u] = start[q[u].x] + q[u].y;
+#pragma unroll
+    for (int u = 0; u < RD_SCATTER_PPL; ++u) ps[d[u]] = v[u];
+    if (slot_of) {
+#pragma unroll
+        for (int u = 0; u < RD_SCATTER_PPL; ++u) slot_of[gid_n[min(s0 + u * RD_BLOCK, n - 1)]] = d[u];
+    }
 }
 
 // replica of each particle index (replica r owns indices [off_r, off_r + n_r),
@@ -1200,7 +1221,8 @@ static void launch_sort(rd_system* h, int advance)
     int nc = h->ncell;
     hipLaunchKernelGGL(k_scan_apply, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum,
                        h->d_start);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(h->n, 1), RD_BLOCK)), dim3(RD_BLOCK), 0, h->stream, h->d_pn,
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(h->n, 1), RD_BLOCK * RD_SCATTER_PPL)), dim3(RD_BLOCK), 0,
+                       h->stream, h->d_pn,
                        h->d_kr, h->d_start, h->d_ps, h->n, h->d_st, advance,
                        (const uint32_t*)h->d_gid, h->d_slot, h->d_bsum, h->nscan);
 }
