@@ -194,7 +194,9 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
-// block-wide exclusive scan of one value per thread; returns the block total
+// block-wide exclusive scan of one value per thread (NT threads); returns
+// the block total
+template <int NT = RD_BLOCK>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* excl, uint32_t* lds)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -203,7 +205,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* excl, 
     __syncthreads();
     uint32_t off = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < RD_BLOCK / 64; ++w) {
+    for (int w = 0; w < NT / 64; ++w) {
         uint32_t t = lds[w];
         if (w < wid) off += t;
         tot += t;
@@ -235,41 +237,46 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_reduce(const uint32_t* __rest
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// Each block adds up the tile sums before it (a few KB, L2-resident), scans
-// its own tile, writes start and zeroes count; the last block writes
-// start[ncell] = total.
-__global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ count, int ncell,
-                                                         const uint32_t* __restrict__ bsum,
-                                                         uint32_t* __restrict__ start)
+// Each block (NT threads, one scan tile) adds up the tile sums before it (a
+// few KB, L2-resident), scans its own tile, writes start and zeroes count;
+// the last block writes start[ncell] = total.  512 threads (4 cells each)
+// halve each thread's share of that prefix against 256 (8 cells each).
+#define RD_SCAN_APPLY_THREADS 512
+template <int NT>
+__global__ __launch_bounds__(NT) void k_scan_apply(uint32_t* __restrict__ count, int ncell,
+                                                   const uint32_t* __restrict__ bsum,
+                                                   uint32_t* __restrict__ start)
 {
-    __shared__ uint32_t lds[RD_BLOCK / 64];
+    constexpr int ITEMS = RD_SCAN_TILE / NT;
+    static_assert(ITEMS % 4 == 0 && ITEMS * NT == RD_SCAN_TILE, "whole 16-byte groups per thread");
+    __shared__ uint32_t lds[NT / 64];
     __shared__ uint32_t pre_sh;
     // this tile's counts are loaded first so that their latency overlaps the
     // sum over the earlier tiles
-    size_t base = (size_t)blockIdx.x * RD_SCAN_TILE + (size_t)threadIdx.x * RD_SCAN_ITEMS;
-    uint32_t c[RD_SCAN_ITEMS];
-    bool full = base + RD_SCAN_ITEMS <= (size_t)ncell;
+    size_t base = (size_t)blockIdx.x * RD_SCAN_TILE + (size_t)threadIdx.x * ITEMS;
+    uint32_t c[ITEMS];
+    bool full = base + ITEMS <= (size_t)ncell;
     if (full) {
         const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
 #pragma unroll
-        for (int k = 0; k < RD_SCAN_ITEMS / 4; ++k) {
+        for (int k = 0; k < ITEMS / 4; ++k) {
             uint4 q = c4[k];
             c[4 * k] = q.x; c[4 * k + 1] = q.y; c[4 * k + 2] = q.z; c[4 * k + 3] = q.w;
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < RD_SCAN_ITEMS; ++k) c[k] = (base + k < (size_t)ncell) ? count[base + k] : 0u;
+        for (int k = 0; k < ITEMS; ++k) c[k] = (base + k < (size_t)ncell) ? count[base + k] : 0u;
     }
     {
         uint32_t acc = 0;
-        for (int i = threadIdx.x; i < (int)blockIdx.x; i += RD_BLOCK) acc += bsum[i];
+        for (int i = threadIdx.x; i < (int)blockIdx.x; i += NT) acc += bsum[i];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
         if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t t = 0;
-            for (int w = 0; w < RD_BLOCK / 64; ++w) t += lds[w];
+            for (int w = 0; w < NT / 64; ++w) t += lds[w];
             pre_sh = t;
             if (blockIdx.x == gridDim.x - 1) start[ncell] = t + bsum[blockIdx.x];
         }
@@ -278,15 +285,15 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ 
     const uint32_t pre = pre_sh;
     uint32_t v = 0;
 #pragma unroll
-    for (int k = 0; k < RD_SCAN_ITEMS; ++k) v += c[k];
+    for (int k = 0; k < ITEMS; ++k) v += c[k];
     uint32_t ex;
-    block_excl_scan(v, &ex, lds);
+    block_excl_scan<NT>(v, &ex, lds);
     uint32_t run = pre + ex;
     if (full) {
         uint4* s4 = reinterpret_cast<uint4*>(start + base);
         uint4* z4 = reinterpret_cast<uint4*>(count + base);
 #pragma unroll
-        for (int k = 0; k < RD_SCAN_ITEMS / 4; ++k) {
+        for (int k = 0; k < ITEMS / 4; ++k) {
             uint4 o;
             o.x = run; run += c[4 * k];
             o.y = run; run += c[4 * k + 1];
@@ -296,7 +303,7 @@ __global__ __launch_bounds__(RD_BLOCK) void k_scan_apply(uint32_t* __restrict__ 
             z4[k] = make_uint4(0u, 0u, 0u, 0u);
         }
     } else {
-        for (int k = 0; k < RD_SCAN_ITEMS; ++k)
+        for (int k = 0; k < ITEMS; ++k)
             if (base + k < (size_t)ncell) { start[base + k] = run; run += c[k]; count[base + k] = 0u; }
     }
 }
@@ -1219,8 +1226,8 @@ static int ensure_capacity(rd_system* h)
 static void launch_sort(rd_system* h, int advance)
 {
     int nc = h->ncell;
-    hipLaunchKernelGGL(k_scan_apply, dim3(h->nscan), dim3(RD_BLOCK), 0, h->stream, h->d_count, nc, h->d_bsum,
-                       h->d_start);
+    hipLaunchKernelGGL(k_scan_apply<RD_SCAN_APPLY_THREADS>, dim3(h->nscan), dim3(RD_SCAN_APPLY_THREADS), 0, h->stream,
+                       h->d_count, nc, h->d_bsum, h->d_start);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(h->n, 1), RD_BLOCK * RD_SCATTER_PPL)), dim3(RD_BLOCK), 0,
                        h->stream, h->d_pn,
                        h->d_kr, h->d_start, h->d_ps, h->n, h->d_st, advance,
